@@ -1,0 +1,176 @@
+"""Headline benchmark: semiring SpMV on a seeded R-MAT scale-22 graph on one MI355X.
+
+One step = one plus.times fp32 SpMV y = A x over the whole graph (R-MAT scale
+22, edge factor 16, (a,b,c) = (.57,.19,.19), duplicates and self-loops
+removed, values 1/coldeg as in the PageRank transition matrix, x uniform in
+[0,1)), in the matrix's current storage format, with A, x and y resident in
+HBM.  `value` is the algorithmic HBM bandwidth of the step,
+
+    bytes/step = 4(n+1) row_ptr + (4+4) nnz col/val + 4n x + 4n y,
+
+divided by the mean step time measured with HIP events on the library's
+stream.  GTEPS (nnz / time) is reported beside it.  The CPU baseline is the
+oracle's OpenMP restatement of the reference's CSR SpMV
+(spmv_csr.h:341-384 with the balancing.h partition) on the host cores, timed
+on the same graph.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 22] [--format csr]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS & HBM GB/s (% peak), semiring SpMV/BFS RMAT-22; batched/served q/s, p99"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--format", default="csr", choices=["csr", "sell_c", "csr_seg", "lav"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the multi-GPU row sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gb_kun_amd as g
+
+    t0 = time.time()
+    rp, col = g.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1, symmetric=False, permute=True)
+    n = len(rp) - 1
+    nnz = len(col)
+    vals = g.inv_coldeg_f32(n, col)
+    x_np = g.uniform_f32(3, n)
+    log(f"graph: R-MAT scale {args.scale} ef {args.edge_factor}: n={n} nnz={nnz} "
+        f"empty rows={int(np.count_nonzero(np.diff(rp) == 0))} max row={int(np.diff(rp).max())} "
+        f"({time.time() - t0:.1f} s)")
+
+    fmt = {"csr": g.FMT_CSR, "sell_c": g.FMT_SELL_C, "csr_seg": g.FMT_CSR_SEG, "lav": g.FMT_LAV}[args.format]
+    t0 = time.time()
+    A = g.Matrix.from_csr(rp, col, vals, fmt=fmt)
+    log(f"device matrix built in {time.time() - t0:.1f} s, {A.device_bytes() / 2**20:.0f} MiB of HBM")
+    cu, arch = g.device_info()
+
+    stream = torch.cuda.current_stream()
+    g.check(g.lib.gbk_set_stream(g._vp(stream.cuda_stream)))
+    x = torch.from_numpy(x_np).cuda()
+    y = torch.zeros(n, dtype=torch.float32, device="cuda")
+
+    def step():
+        g.spmv_device(A, x.data_ptr(), y.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # correctness spot check of the timed configuration against FP64
+    from oracle import oracle as O
+    y_np = y.cpu().numpy()
+    rows = np.random.default_rng(0).choice(n, size=min(n, 20000), replace=False)
+    want, absum = O.spmv_f64(rp, col, vals, x_np)
+    err = np.max(np.abs(y_np[rows].astype(np.float64) - want[rows]) / np.maximum(absum[rows], 1e-30))
+    log(f"spot check vs FP64 on {len(rows)} rows: max rel err {err:.2e}")
+
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = ev0.elapsed_time(ev1) / args.steps
+
+    algo_bytes = 4 * (n + 1) + 8 * nnz + 4 * n + 4 * n
+    ref_bytes = 16 * nnz
+    gbs = algo_bytes / (ms * 1e-3) / 1e9
+    gteps = nnz / (ms * 1e-3) / 1e9
+    log(f"SpMV {ms * 1e3:.1f} us/step  {gbs:.0f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  "
+        f"{gteps:.1f} GTEPS  wall {wall * 1e3 / args.steps:.1f} us/step")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_spmv_{args.format}_rmat{args.scale}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(O, rp, col, vals, x_np, algo_bytes, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded R-MAT, values 1/coldeg)",
+        "config": {"workload": f"plus.times fp32 SpMV, R-MAT scale {args.scale}, edge factor {args.edge_factor}, "
+                               f"{args.format} format", "n": n, "nnz": nnz, "format": args.format,
+                   "parallelism": "single GPU"},
+        "gteps": round(gteps, 3),
+        "ref_accounting_gbs": round(ref_bytes / (ms * 1e-3) / 1e9, 2),
+        "device": {"arch": arch, "cus": cu},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_spmv_merge + k_spmv_merge_fixup (one SpMV)",
+                     "algorithmic_bytes_per_launch": algo_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(O, rp, col, vals, x, algo_bytes, seconds):
+    """Times the oracle's OpenMP CSR SpMV (the reference's partition and fold) on the host."""
+    nthreads = int(os.environ.get("GBK_CPU_THREADS", "0")) or len(os.sched_getaffinity(0))
+    nthreads = min(nthreads, 16)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    first, last = O.balance_rows(rp, nthreads)
+    rp64 = np.ascontiguousarray(rp, dtype=np.int64)
+    col32 = np.ascontiguousarray(col, dtype=np.int32)
+    y = np.zeros(len(x), dtype=np.float32)
+    O.spmv_f32_omp(rp64, col32, vals, x, y, nthreads, first, last)  # warm-up
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end and len(times) < 200:
+        t0 = time.perf_counter()
+        O.spmv_f32_omp(rp64, col32, vals, x, y, nthreads, first, last)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    gbs = algo_bytes / med / 1e9
+    log(f"CPU baseline ({nthreads} threads): median {med * 1e3:.2f} ms over {len(times)} runs, {gbs:.1f} GB/s")
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "sample": f"full graph, median of {len(times)} SpMV runs ({med * 1e3:.2f} ms each)",
+            "ms_per_spmv": round(med * 1e3, 3)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,376 @@
+"""Python binding of libgbk.so, the MI355X (gfx950) semiring SpMV / SpMSpV library.
+
+The binding is ctypes over the C ABI declared in include/gbk.h; it adds numpy
+conversion and object lifetimes, nothing else.  Every computation runs in the
+HIP kernels of libgbk.so: there is no CPU fallback, and importing this package
+fails if the library has not been built (python -m gb_kun_amd.build_lib, or
+__graft_entry__.build()).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgbk.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run python -m gb_kun_amd.build_lib")
+
+lib = C.CDLL(LIB_PATH)
+
+# ---------------------------------------------------------------- constants
+SUCCESS, NO_VALUE, UNINITIALIZED_OBJECT, INVALID_OBJECT, NULL_POINTER = 0, 1, 2, 3, 4
+INVALID_VALUE, INVALID_INDEX, DOMAIN_MISMATCH, DIMENSION_MISMATCH = 5, 6, 7, 8
+OUT_OF_MEMORY, INDEX_OUT_OF_BOUNDS, PANIC = 10, 12, 13
+
+FP32, FP64, INT32, INT64, BOOL = 0, 1, 2, 3, 4
+NP_TYPES = {FP32: np.float32, FP64: np.float64, INT32: np.int32, INT64: np.int64, BOOL: np.uint8}
+TYPE_OF = {np.dtype(v): k for k, v in NP_TYPES.items()}
+
+ADD_PLUS, ADD_LOR, ADD_MIN, ADD_MAX, ADD_FIRST, ADD_SECOND, ADD_LESS, ADD_FIRSTMIN = range(8)
+(MUL_TIMES, MUL_LAND, MUL_FIRST, MUL_SECOND, MUL_PLUS, MUL_ONEB, MUL_NE, MUL_MINUS,
+ MUL_DIV) = range(9)
+(ACC_NONE, ACC_FIRST, ACC_PLUS, ACC_MINUS, ACC_TIMES, ACC_DIV, ACC_MIN, ACC_MAX, ACC_LOR,
+ ACC_LAND, ACC_LESS, ACC_NE) = range(12)
+
+SCMP, REPLACE, COMP, TRAN, STRUCTURE, STR_COMP, DEFAULT = 0, 1, 2, 3, 4, 5, 6
+FMT_CSR, FMT_CSR_SEG, FMT_LAV, FMT_SELL_C = 0, 3, 4, 6
+STORAGE_SPARSE, STORAGE_DENSE = 1, 2
+
+
+class Semiring(C.Structure):
+    _fields_ = [("add", C.c_int), ("mul", C.c_int)]
+
+
+class Desc(C.Structure):
+    _fields_ = [("mask", C.c_int), ("output", C.c_int)]
+
+
+# the semirings registered by the reference (src/cpp_graphblas/types.hpp:274-284)
+SEMIRINGS = {
+    "PlusMultiplies": (ADD_PLUS, MUL_TIMES),
+    "LogicalOrAnd": (ADD_LOR, MUL_LAND),
+    "FirstWins": (ADD_FIRST, MUL_TIMES),
+    "FirstMin": (ADD_FIRSTMIN, MUL_TIMES),
+    "PlusSecond": (ADD_PLUS, MUL_SECOND),
+    "StructuralBool": (ADD_LOR, MUL_ONEB),
+    "MinimumPlus": (ADD_MIN, MUL_PLUS),
+    "CustomLessPlus": (ADD_LESS, MUL_PLUS),
+    "MinimumSelectSecond": (ADD_MIN, MUL_SECOND),
+    "MinimumNotEqualTo": (ADD_MIN, MUL_NE),
+    "PlusOne": (ADD_PLUS, MUL_ONEB),
+}
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_pi64 = C.POINTER(C.c_int64)
+_pi32 = C.POINTER(C.c_int32)
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("gbk_init", C.c_int, C.c_int)
+_sig("gbk_set_stream", C.c_int, _vp)
+_sig("gbk_get_stream", _vp)
+_sig("gbk_synchronize", C.c_int)
+_sig("gbk_last_error", C.c_char_p)
+_sig("gbk_device_info", C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int)
+_sig("gbk_matrix_new", C.c_int, C.POINTER(_vp), C.c_int)
+_sig("gbk_matrix_free", C.c_int, _vp)
+_sig("gbk_matrix_set_format", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
+_sig("gbk_matrix_get_format", C.c_int, _vp, C.POINTER(C.c_int))
+_sig("gbk_matrix_build_coo", C.c_int, _vp, _pi64, _pi64, _vp, _i64)
+_sig("gbk_matrix_build_csr", C.c_int, _vp, _pi64, _pi64, _vp, _i64, _i64)
+_sig("gbk_matrix_build_csr32", C.c_int, _vp, _pi64, _pi32, _vp, _i64, _i64)
+_sig("gbk_matrix_nrows", C.c_int, _vp, _pi64)
+_sig("gbk_matrix_nvals", C.c_int, _vp, _pi64)
+_sig("gbk_matrix_export", C.c_int, _vp, C.c_int, _pi64, _pi32, _vp)
+_sig("gbk_matrix_degrees", C.c_int, _vp, C.c_int, _pi64)
+_sig("gbk_matrix_device_bytes", C.c_int, _vp, _pi64)
+_sig("gbk_vector_new", C.c_int, C.POINTER(_vp), C.c_int, _i64)
+_sig("gbk_vector_free", C.c_int, _vp)
+_sig("gbk_vector_nvals", C.c_int, _vp, _pi64)
+_sig("gbk_vector_storage", C.c_int, _vp, C.POINTER(C.c_int))
+_sig("gbk_vector_build_dense", C.c_int, _vp, _vp, _i64)
+_sig("gbk_vector_build_sparse", C.c_int, _vp, _pi64, _vp, _i64)
+_sig("gbk_vector_fill", C.c_int, _vp, C.c_double)
+_sig("gbk_vector_fill_int", C.c_int, _vp, _i64)
+_sig("gbk_vector_set_element", C.c_int, _vp, C.c_double, _i64)
+_sig("gbk_vector_fill_ascending", C.c_int, _vp, _i64)
+_sig("gbk_vector_dup", C.c_int, _vp, _vp)
+_sig("gbk_vector_swap", C.c_int, _vp, _vp)
+_sig("gbk_vector_clear", C.c_int, _vp)
+_sig("gbk_vector_force_dense", C.c_int, _vp)
+_sig("gbk_vector_force_sparse", C.c_int, _vp)
+_sig("gbk_vector_convert_if_required", C.c_int, _vp)
+_sig("gbk_vector_to_host", C.c_int, _vp, _vp)
+_sig("gbk_vector_sparse_to_host", C.c_int, _vp, _pi64, _vp, _pi64)
+_sig("gbk_vector_device_dense", C.c_int, _vp, C.POINTER(_vp))
+_sig("gbk_mxv", C.c_int, _vp, _vp, C.c_int, Semiring, _vp, _vp, C.POINTER(Desc))
+_sig("gbk_vxm", C.c_int, _vp, _vp, C.c_int, Semiring, _vp, _vp, C.POINTER(Desc))
+_sig("gbk_spmv_device", C.c_int, _vp, C.c_int, _vp, _vp, Semiring, C.c_int)
+_sig("gbk_gen_rmat", C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
+     C.c_int, _pi64, _pi64, C.POINTER(_pi64), C.POINTER(_pi32))
+_sig("gbk_gen_uniform", C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _pi64, _pi64, C.POINTER(_pi64),
+     C.POINTER(_pi32))
+_sig("gbk_host_free", None, _vp)
+_sig("gbk_fill_uniform_f32", None, C.c_uint64, _i64, _vp)
+_sig("gbk_fill_inv_coldeg_f32", None, _i64, _i64, _pi32, _vp)
+_sig("gbk_fill_sym_weights_i32", None, _i64, _pi64, _pi32, C.c_uint64, C.c_int32, C.c_int32, _vp)
+
+
+class GbkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gbk error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != SUCCESS:
+        raise GbkError(rc, lib.gbk_last_error().decode(errors="replace"))
+    return rc
+
+
+def _ptr(a, ctype=None):
+    if a is None:
+        return None
+    if ctype is None:
+        return a.ctypes.data_as(_vp)
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def device_info():
+    cu = C.c_int(0)
+    name = C.create_string_buffer(128)
+    check(lib.gbk_device_info(C.byref(cu), name, 128))
+    return cu.value, name.value.decode()
+
+
+def synchronize():
+    check(lib.gbk_synchronize())
+
+
+# ------------------------------------------------------------- generators
+def _take_csr(n, nnz, rp, col):
+    rp_np = np.ctypeslib.as_array(rp, shape=(n.value + 1,)).copy()
+    col_np = np.ctypeslib.as_array(col, shape=(max(nnz.value, 1),))[: nnz.value].copy()
+    lib.gbk_host_free(C.cast(rp, _vp))
+    lib.gbk_host_free(C.cast(col, _vp))
+    return rp_np, col_np
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, symmetric=False, permute=True):
+    """Seeded R-MAT graph as a CSR (row_ptr int64[n+1], col int32[nnz]): duplicates and self-loops removed."""
+    n, nnz = _i64(0), _i64(0)
+    rp, col = _pi64(), _pi32()
+    check(lib.gbk_gen_rmat(scale, edge_factor, a, b, c, seed, int(symmetric), int(permute), C.byref(n),
+                           C.byref(nnz), C.byref(rp), C.byref(col)))
+    return _take_csr(n, nnz, rp, col)
+
+
+def gen_uniform(scale, edge_factor=16, seed=1, symmetric=False):
+    """Seeded Erdos-Renyi (uniform) graph as a CSR, duplicates and self-loops removed."""
+    n, nnz = _i64(0), _i64(0)
+    rp, col = _pi64(), _pi32()
+    check(lib.gbk_gen_uniform(scale, edge_factor, seed, int(symmetric), C.byref(n), C.byref(nnz), C.byref(rp),
+                              C.byref(col)))
+    return _take_csr(n, nnz, rp, col)
+
+
+def uniform_f32(seed, count):
+    out = np.empty(count, dtype=np.float32)
+    lib.gbk_fill_uniform_f32(seed, count, _ptr(out))
+    return out
+
+
+def inv_coldeg_f32(n, col):
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    out = np.empty(len(col), dtype=np.float32)
+    lib.gbk_fill_inv_coldeg_f32(n, len(col), _ptr(col, C.c_int32), _ptr(out))
+    return out
+
+
+def sym_weights_i32(row_ptr, col, seed, lo=1, hi=255):
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    out = np.empty(len(col), dtype=np.int32)
+    lib.gbk_fill_sym_weights_i32(len(row_ptr) - 1, _ptr(row_ptr, C.c_int64), _ptr(col, C.c_int32), seed, lo, hi,
+                                 _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- objects
+class Matrix:
+    """Device matrix (CSR + CSC in HBM), lablas::Matrix<T> counterpart."""
+
+    def __init__(self, type_=FP32):
+        self.h = _vp()
+        self.type = type_
+        check(lib.gbk_matrix_new(C.byref(self.h), type_))
+
+    @classmethod
+    def from_csr(cls, row_ptr, col, vals, fmt=None, sell_c=0, sigma=0):
+        vals = np.ascontiguousarray(vals)
+        m = cls(TYPE_OF[vals.dtype])
+        if fmt is not None:
+            m.set_format(fmt, sell_c, sigma)
+        m.build_csr(row_ptr, col, vals)
+        return m
+
+    def set_format(self, fmt, sell_c=0, sigma=0):
+        check(lib.gbk_matrix_set_format(self.h, fmt, sell_c, sigma))
+
+    def build_csr(self, row_ptr, col, vals):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        vals = np.ascontiguousarray(vals, dtype=NP_TYPES[self.type])
+        nrows = len(row_ptr) - 1
+        if np.asarray(col).dtype == np.int32:
+            col = np.ascontiguousarray(col)
+            check(lib.gbk_matrix_build_csr32(self.h, _ptr(row_ptr, C.c_int64), _ptr(col, C.c_int32), _ptr(vals),
+                                             nrows, len(col)))
+        else:
+            col = np.ascontiguousarray(col, dtype=np.int64)
+            check(lib.gbk_matrix_build_csr(self.h, _ptr(row_ptr, C.c_int64), _ptr(col, C.c_int64), _ptr(vals),
+                                           nrows, len(col)))
+
+    def build_coo(self, rows, cols, vals):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        vals = np.ascontiguousarray(vals, dtype=NP_TYPES[self.type])
+        check(lib.gbk_matrix_build_coo(self.h, _ptr(rows, C.c_int64), _ptr(cols, C.c_int64), _ptr(vals), len(rows)))
+
+    @property
+    def n(self):
+        v = _i64(0)
+        check(lib.gbk_matrix_nrows(self.h, C.byref(v)))
+        return v.value
+
+    @property
+    def nnz(self):
+        v = _i64(0)
+        check(lib.gbk_matrix_nvals(self.h, C.byref(v)))
+        return v.value
+
+    def export(self, transposed=False):
+        n, nnz = self.n, self.nnz
+        rp = np.empty(n + 1, dtype=np.int64)
+        col = np.empty(max(nnz, 1), dtype=np.int32)
+        vals = np.empty(max(nnz, 1), dtype=NP_TYPES[self.type])
+        check(lib.gbk_matrix_export(self.h, int(transposed), _ptr(rp, C.c_int64), _ptr(col, C.c_int32), _ptr(vals)))
+        return rp, col[:nnz], vals[:nnz]
+
+    def device_bytes(self):
+        v = _i64(0)
+        check(lib.gbk_matrix_device_bytes(self.h, C.byref(v)))
+        return v.value
+
+    def free(self):
+        if self.h:
+            lib.gbk_matrix_free(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Vector:
+    """Device vector with the reference's sparse/dense switch (lablas::Vector<T>)."""
+
+    def __init__(self, n, type_=FP32):
+        self.h = _vp()
+        self.type = type_
+        self.n = n
+        check(lib.gbk_vector_new(C.byref(self.h), type_, n))
+
+    @classmethod
+    def dense(cls, vals):
+        vals = np.ascontiguousarray(vals)
+        v = cls(len(vals), TYPE_OF[vals.dtype])
+        check(lib.gbk_vector_build_dense(v.h, _ptr(vals), len(vals)))
+        return v
+
+    @classmethod
+    def sparse(cls, n, ids, vals):
+        vals = np.ascontiguousarray(vals)
+        v = cls(n, TYPE_OF[vals.dtype])
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        check(lib.gbk_vector_build_sparse(v.h, _ptr(ids, C.c_int64), _ptr(vals), len(ids)))
+        return v
+
+    def fill(self, value):
+        check(lib.gbk_vector_fill(self.h, float(value)))
+
+    def set_element(self, value, pos):
+        check(lib.gbk_vector_set_element(self.h, float(value), pos))
+
+    @property
+    def nvals(self):
+        v = _i64(0)
+        check(lib.gbk_vector_nvals(self.h, C.byref(v)))
+        return v.value
+
+    @property
+    def storage(self):
+        s = C.c_int(0)
+        check(lib.gbk_vector_storage(self.h, C.byref(s)))
+        return s.value
+
+    def to_numpy(self):
+        out = np.empty(self.n, dtype=NP_TYPES[self.type])
+        check(lib.gbk_vector_to_host(self.h, _ptr(out)))
+        return out
+
+    def sparse_to_numpy(self):
+        ids = np.empty(max(self.n, 1), dtype=np.int64)
+        vals = np.empty(max(self.n, 1), dtype=NP_TYPES[self.type])
+        nv = _i64(0)
+        check(lib.gbk_vector_sparse_to_host(self.h, _ptr(ids, C.c_int64), _ptr(vals), C.byref(nv)))
+        return ids[: nv.value], vals[: nv.value]
+
+    def force_dense(self):
+        check(lib.gbk_vector_force_dense(self.h))
+
+    def force_sparse(self):
+        check(lib.gbk_vector_force_sparse(self.h))
+
+    def free(self):
+        if self.h:
+            lib.gbk_vector_free(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _desc(mask=DEFAULT, output=DEFAULT):
+    return C.byref(Desc(mask, output))
+
+
+def mxv(w, mask, accum, semiring, A, u, desc_mask=DEFAULT):
+    add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
+    check(lib.gbk_mxv(w.h, mask.h if mask is not None else None, accum, Semiring(add, mul), A.h, u.h,
+                      _desc(desc_mask)))
+
+
+def vxm(w, mask, accum, semiring, u, A, desc_mask=DEFAULT):
+    add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
+    check(lib.gbk_vxm(w.h, mask.h if mask is not None else None, accum, Semiring(add, mul), u.h, A.h,
+                      _desc(desc_mask)))
+
+
+def spmv_device(A, x_ptr, y_ptr, semiring=(ADD_PLUS, MUL_TIMES), accum=ACC_NONE, transposed=False):
+    """y = accum(y, A (+.*) x) on raw device pointers (e.g. torch tensor data_ptr())."""
+    add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
+    check(lib.gbk_spmv_device(A.h, int(transposed), _vp(x_ptr), _vp(y_ptr), Semiring(add, mul), accum))

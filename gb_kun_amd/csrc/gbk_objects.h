@@ -1,0 +1,114 @@
+// Device-resident matrix and vector objects and the internal entry points
+// the C ABI is built from.
+//
+// Layout in HBM (per matrix, per orientation — CSR rows and CSC columns):
+//   row_ptr  int32[n+1]        nnz < 2^31 is required
+//   col      int32[nnz]
+//   val      T[nnz]
+//   tiles    int2[ntiles+1]    merge-path start coordinates (row, nz) of
+//                              every 2048-item tile of the SpMV
+//   carry    T[2*ntiles]       per-tile partial sums of rows that cross tiles
+// plus the optional optimized representations (SELL-C-sigma, segmented CSR,
+// LAV) and a workspace of n-sized scratch vectors.
+#pragma once
+
+#include "gbk_common.h"
+
+namespace gbk {
+
+struct SellC;
+struct SegCSR;
+struct Lav;
+
+struct DevCSR {
+    int64_t nrows = 0, ncols = 0, nnz = 0;
+    int32_t* row_ptr = nullptr;
+    int32_t* col = nullptr;
+    void* val = nullptr;
+    // merge-path tiling
+    int64_t ntiles = 0;
+    int2* tiles = nullptr;
+    void* carry = nullptr;  // 2*ntiles elements of the value type (tail, head)
+    // row-length statistics (for kernel selection)
+    int32_t max_row = 0;
+};
+
+struct Workspace {
+    void* buf[4] = {nullptr, nullptr, nullptr, nullptr};  // n * 8 bytes each
+    size_t bytes = 0;
+};
+
+}  // namespace gbk
+
+struct gbk_matrix_s {
+    int type = GBK_FP32;
+    int64_t n = 0;    // square: nrows == ncols
+    int64_t nnz = 0;
+    bool built = false;
+    int format = GBK_FMT_CSR;
+    int sell_c = 64, sigma = 512;
+    gbk::DevCSR csr, csc;
+    gbk::SellC* sell[2] = {nullptr, nullptr};
+    gbk::SegCSR* seg[2] = {nullptr, nullptr};
+    gbk::Lav* lav[2] = {nullptr, nullptr};
+    gbk::Workspace ws;
+};
+
+struct gbk_vector_s {
+    int type = GBK_FP32;
+    int64_t n = 0;
+    int storage = GBK_STORAGE_SPARSE;
+    void* dense = nullptr;     // T[n]
+    int32_t* ids = nullptr;    // int32[n] (sparse ids, ascending after compaction)
+    void* svals = nullptr;     // T[n]
+    int64_t nvals = 0;         // entries of the sparse form
+};
+
+namespace gbk {
+
+// runtime.hip
+int ensure_init();
+hipStream_t stream();
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+int device_cu_count();
+// scratch buffers owned by the library (grown on demand, reused across calls)
+int scratch(int slot, size_t bytes, void** p);
+
+// scans / compaction (runtime.hip)
+// out[i] = sum_{j<i} in[j] for i in [0, n]; returns total through *total
+// (host) if total != nullptr.  in may be nullptr with deg given by row_ptr
+// differences of the frontier (see spmspv).
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* total_host);
+// Dense -> sparse: ids/vals of nonzero entries, ascending.  Returns count.
+int compact_nonzero(int type, const void* dense, int64_t n, int32_t* ids, void* vals, int64_t* count);
+int count_nonzero(int type, const void* dense, int64_t n, int64_t* count);
+int fill_value(int type, void* dst, int64_t n, double value);
+int fill_identity(int type, int add, void* dst, int64_t n);
+int scatter_sparse(int type, const int32_t* ids, const void* vals, int64_t nvals, void* dense);
+
+// vector state (vector.hip)
+int vec_force_dense(gbk_vector v);
+int vec_force_sparse(gbk_vector v);
+int vec_convert_if_required(gbk_vector v);
+int vec_nvals(gbk_vector v, int64_t* nvals);
+
+// spmv.hip
+int spmv_unmasked(gbk_matrix A, int transposed, const void* x, void* y, int add, int mul, int accum);
+int spmv_masked_dense(gbk_matrix A, int transposed, const void* x, void* y, int add, int mul, int accum,
+                      const void* mask, int mask_type, bool comp);
+int spmv_masked_rows(gbk_matrix A, int transposed, const void* x, void* y, int add, int mul, int accum,
+                     const int32_t* rows, int64_t nrows_list);
+int build_merge_tiles(DevCSR& m, int type);
+
+// spmspv.hip
+int spmspv(gbk_matrix A, bool use_csr, const int32_t* x_ids, const void* x_vals, int64_t x_nvals,
+           void* y, int add, int mul, int accum, gbk_vector mask, int mask_mode);
+
+// formats (formats.hip)
+int format_build(gbk_matrix A);
+void format_free(gbk_matrix A);
+int format_spmv(gbk_matrix A, int transposed, const void* x, void* y, int add, int mul, int accum, bool* handled);
+int64_t format_bytes(gbk_matrix A);
+
+}  // namespace gbk
