@@ -93,6 +93,7 @@ _sig("gbk_matrix_nvals", C.c_int, _vp, _pi64)
 _sig("gbk_matrix_export", C.c_int, _vp, C.c_int, _pi64, _pi32, _vp)
 _sig("gbk_matrix_degrees", C.c_int, _vp, C.c_int, _pi64)
 _sig("gbk_matrix_device_bytes", C.c_int, _vp, _pi64)
+_sig("gbk_matrix_format_stats", C.c_int, _vp, C.c_int, _pi64)
 _sig("gbk_vector_new", C.c_int, C.POINTER(_vp), C.c_int, _i64)
 _sig("gbk_vector_free", C.c_int, _vp)
 _sig("gbk_vector_nvals", C.c_int, _vp, _pi64)
@@ -115,6 +116,20 @@ _sig("gbk_vector_device_dense", C.c_int, _vp, C.POINTER(_vp))
 _sig("gbk_mxv", C.c_int, _vp, _vp, C.c_int, Semiring, _vp, _vp, C.POINTER(Desc))
 _sig("gbk_vxm", C.c_int, _vp, _vp, C.c_int, Semiring, _vp, _vp, C.POINTER(Desc))
 _sig("gbk_spmv_device", C.c_int, _vp, C.c_int, _vp, _vp, Semiring, C.c_int)
+_sig("gbk_assign_scalar", C.c_int, _vp, _vp, C.c_int, C.c_double, C.POINTER(Desc))
+_sig("gbk_assign_vector", C.c_int, _vp, _vp, C.c_int, _vp, C.POINTER(Desc))
+_sig("gbk_assign_scatter", C.c_int, _vp, _vp, C.c_int, _vp, _vp, _i64, C.POINTER(Desc))
+_sig("gbk_ewise_add", C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(Desc))
+_sig("gbk_ewise_mult", C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(Desc))
+_sig("gbk_apply", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp, C.POINTER(Desc))
+_sig("gbk_reduce", C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int, _vp, C.POINTER(Desc))
+_sig("gbk_reduce_i64", C.c_int, _pi64, C.c_int, C.c_int, _vp, C.POINTER(Desc))
+_sig("gbk_extract", C.c_int, _vp, _vp, C.c_int, _vp, _vp, C.POINTER(Desc))
+_sig("gbk_bfs", C.c_int, _vp, _i64, _pi32, _pi64)
+_sig("gbk_bfs_device", C.c_int, _vp, _i64, _vp, _pi64)
+_sig("gbk_sssp", C.c_int, _vp, _i64, _vp, _pi64)
+_sig("gbk_cc", C.c_int, _vp, _pi32, _pi64)
+_sig("gbk_pagerank", C.c_int, _vp, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
 _sig("gbk_gen_rmat", C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
      C.c_int, _pi64, _pi64, C.POINTER(_pi64), C.POINTER(_pi32))
 _sig("gbk_gen_uniform", C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _pi64, _pi64, C.POINTER(_pi64),
@@ -270,6 +285,12 @@ class Matrix:
         check(lib.gbk_matrix_device_bytes(self.h, C.byref(v)))
         return v.value
 
+    def format_stats(self, transposed=False):
+        """(padded SELL-C entries, problematic-chunk rows, segments, segment-row pairs)."""
+        st = np.zeros(4, dtype=np.int64)
+        check(lib.gbk_matrix_format_stats(self.h, int(transposed), _ptr(st, C.c_int64)))
+        return tuple(int(v) for v in st)
+
     def free(self):
         if self.h:
             lib.gbk_matrix_free(self.h)
@@ -374,3 +395,82 @@ def spmv_device(A, x_ptr, y_ptr, semiring=(ADD_PLUS, MUL_TIMES), accum=ACC_NONE,
     """y = accum(y, A (+.*) x) on raw device pointers (e.g. torch tensor data_ptr())."""
     add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
     check(lib.gbk_spmv_device(A.h, int(transposed), _vp(x_ptr), _vp(y_ptr), Semiring(add, mul), accum))
+
+
+# ------------------------------------------------------------ graph drivers
+def bfs(A, source):
+    """Direction-optimizing BFS: int32 levels (1-based, 0 unreached) and stats
+    (levels, push steps, pull steps, edges traversed)."""
+    lv = np.empty(A.n, dtype=np.int32)
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_bfs(A.h, source, _ptr(lv, C.c_int32), _ptr(st, C.c_int64)))
+    return lv, st[:4]
+
+
+def sssp(A, source):
+    """min.plus SSSP distances in the matrix type (INT32: INT32_MAX unreached, FP32: FLT_MAX)."""
+    d = np.empty(A.n, dtype=NP_TYPES[A.type])
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_sssp(A.h, source, _ptr(d), _ptr(st, C.c_int64)))
+    return d, st[:2]
+
+
+def cc(A):
+    """Connected-component labels (minimum vertex id of each component) of a symmetric matrix."""
+    lab = np.empty(A.n, dtype=np.int32)
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_cc(A.h, _ptr(lab, C.c_int32), _ptr(st, C.c_int64)))
+    return lab, st[:1]
+
+
+def pagerank(A, itermax=100, damping=0.85, tol=0.0):
+    """PageRank with sinks (fp32); returns (ranks, iterations run)."""
+    r = np.empty(A.n, dtype=np.float32)
+    it = C.c_int(0)
+    check(lib.gbk_pagerank(A.h, _ptr(r, C.c_float), itermax, damping, tol, C.byref(it)))
+    return r, it.value
+
+
+# ------------------------------------------------------- element-wise ops
+def _mh(v):
+    return v.h if v is not None else None
+
+
+def assign_scalar(w, mask, value, desc_mask=DEFAULT, accum=ACC_NONE):
+    check(lib.gbk_assign_scalar(w.h, _mh(mask), accum, float(value), _desc(desc_mask)))
+
+
+def assign_vector(w, mask, u, accum=ACC_NONE, desc_mask=DEFAULT):
+    check(lib.gbk_assign_vector(w.h, _mh(mask), accum, u.h, _desc(desc_mask)))
+
+
+def assign_scatter(w, mask, u, indices, nindices, accum=ACC_NONE):
+    check(lib.gbk_assign_scatter(w.h, _mh(mask), accum, u.h, indices.h, nindices, _desc()))
+
+
+def ewise_add(w, mask, op, u, v):
+    check(lib.gbk_ewise_add(w.h, _mh(mask), ACC_NONE, op, u.h, v.h, _desc()))
+
+
+def ewise_mult(w, mask, op, u, v):
+    check(lib.gbk_ewise_mult(w.h, _mh(mask), ACC_NONE, op, u.h, v.h, _desc()))
+
+
+def apply(w, mask, unary, u, scalar=0.0):
+    check(lib.gbk_apply(w.h, _mh(mask), ACC_NONE, unary, float(scalar), u.h, _desc()))
+
+
+def reduce(u, monoid=ADD_PLUS, init=0.0, accum=ACC_PLUS):
+    r = C.c_double(init)
+    check(lib.gbk_reduce(C.byref(r), accum, monoid, u.h, _desc()))
+    return r.value
+
+
+def reduce_i64(u, monoid=ADD_PLUS, init=0, accum=ACC_PLUS):
+    r = C.c_int64(init)
+    check(lib.gbk_reduce_i64(C.byref(r), accum, monoid, u.h, _desc()))
+    return r.value
+
+
+def extract(w, mask, u, indices, accum=ACC_NONE):
+    check(lib.gbk_extract(w.h, _mh(mask), accum, u.h, _mh(indices), _desc()))

@@ -95,7 +95,25 @@ def build(verbose=True, jobs=None):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB, flush=True)
+    build_cpp_drivers(verbose)
     return LIB
+
+
+def build_cpp_drivers(verbose=True):
+    """Compiles the C++ front-end drivers under tests/cpp against include/gbk/lablas.hpp and libgbk.so."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "bin")
+    if not os.path.isdir(src_dir):
+        return
+    os.makedirs(out_dir, exist_ok=True)
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, f[:-4])
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(src_dir, f),
+              "-L", HERE, "-lgbk", "-Wl,-rpath,$ORIGIN/../../../gb_kun_amd", "-o", exe])
+        if verbose:
+            print("built", exe, flush=True)
 
 
 if __name__ == "__main__":

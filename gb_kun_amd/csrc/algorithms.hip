@@ -1,0 +1,551 @@
+// Graph drivers on top of the SpMV / SpMSpV path, with the element-wise
+// passes of each iteration fused into the traversal kernels.
+//
+//   BFS  — replaces bfs_blast (algorithms/bfs/bfs.hpp:94-142).  The
+//          reference alternates push (SpMSpV over the CSR + complement mask)
+//          and pull (masked SpMV over the CSC) by the 30% density of the
+//          frontier vector.  Here the switch is Beamer's edge heuristic:
+//          push while the frontier's out-edges m_f < m_u / 14 (m_u = edges
+//          of unvisited vertices), pull while the frontier has more than
+//          n / 24 vertices.  Push: edge-balanced expansion over the frontier
+//          queue, atomicCAS claims a vertex, wave-aggregated append to the
+//          next queue.  Pull: one lane group per unvisited vertex scans its
+//          in-edges and stops at the first parent in the frontier.  Levels
+//          are 1-based and 0 means unreached, as in the reference; they do
+//          not depend on the direction chosen.
+//   SSSP — replaces sssp_bellman_ford_blast (sssp_blast.hpp:8-72): the same
+//          fixed point (min.plus relaxation until no distance improves),
+//          computed over the set of vertices improved in the last round
+//          instead of all vertices.  int32 distances saturate at INT32_MAX.
+//   CC   — replaces FastSV cc (cc.hpp:12-119): min-label hooking over the
+//          edges plus pointer jumping, which converges to the same labels
+//          (the minimum vertex id of each component).
+//   PR   — replaces LAGraph_page_rank_sinks (pr.hpp:20-153): r = teleport +
+//          A (+.second) (t ./ d), d = max(coldeg / damping, 1 / damping),
+//          teleport = (1 - damping) / n + damping / n * sum of the sinks'
+//          ranks, with the SpMV on the merge-path kernel.
+#include <cfloat>
+#include <vector>
+
+#include "gbk_objects.h"
+
+namespace gbk {
+
+int spmv_csr_unmasked(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum);
+
+static dim3 grid_n(int64_t n, int per = 256) {
+    int64_t b = (n + per - 1) / per;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt64() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Appends v to queue (wave-aggregated: one atomic per wave).
+__device__ __forceinline__ void wave_append(bool take, int32_t v, int32_t* __restrict__ queue, int32_t* __restrict__ count) {
+    unsigned long long b = __ballot(take);
+    if (b == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)b) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, (int)__popcll(b));
+    base = __shfl(base, leader, 64);
+    if (take) queue[base + __popcll(b & lanemask_lt64())] = v;
+}
+
+// ------------------------------------------------------------------- BFS --
+__global__ void k_frontier_deg(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
+                               int32_t* __restrict__ deg) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        int u = q[i];
+        deg[i] = rp[u + 1] - rp[u];
+    }
+}
+
+// Push: thread e owns edge e of the frontier's concatenated adjacency lists.
+__global__ __launch_bounds__(256) void k_bfs_push(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ q, int64_t nq,
+                                                  const int32_t* __restrict__ offs, int64_t nedges,
+                                                  int32_t* __restrict__ level, int32_t next_level,
+                                                  int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    int32_t v = 0;
+    if (e < nedges) {
+        int64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const int u = q[lo];
+        v = col[rp[u] + (e - offs[lo])];
+        if (level[v] == 0) take = (atomicCAS(&level[v], 0, next_level) == 0);
+    }
+    wave_append(take, v, nq_out, nq_count);
+}
+
+// Pull: G lanes per vertex; an unvisited vertex joins the next frontier if
+// any in-neighbour is in the current one (level == cur).
+template <int G>
+__global__ __launch_bounds__(256) void k_bfs_pull(const int32_t* __restrict__ crp, const int32_t* __restrict__ ccol,
+                                                  int64_t n, int32_t* __restrict__ level, int32_t cur,
+                                                  int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    bool found = false;
+    if (v < n && level[v] == 0) {
+        const int e1 = crp[v + 1];
+        for (int e = crp[v] + lane; e < e1; e += G) {
+            if (level[ccol[e]] == cur) {
+                found = true;
+                break;
+            }
+        }
+        if constexpr (G > 1) {
+            // any lane of the group
+            unsigned long long b = __ballot(found);
+            const int gbase = (threadIdx.x & 63) & ~(G - 1);
+            const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
+            found = (b & gmask) != 0;
+        }
+    }
+    const bool take = found && lane == 0;
+    if (take) level[v] = cur + 1;
+    wave_append(take, (int32_t)v, nq_out, nq_count);
+}
+
+__global__ void k_set_level(int32_t* level, int64_t s, int32_t* q, int32_t* count) {
+    level[s] = 1;
+    q[0] = (int32_t)s;
+    *count = 1;
+}
+
+__global__ void k_queue_edges(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
+                              unsigned long long* __restrict__ sum) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long d = 0;
+    if (i < nq) {
+        int u = q[i];
+        d = (unsigned long long)(rp[u + 1] - rp[u]);
+    }
+    // wave reduce then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+    if ((threadIdx.x & 63) == 0 && d) atomicAdd(sum, d);
+}
+
+struct BfsScratch {
+    int32_t* q[2];
+    int32_t* cnt;  // [2] counters
+    int32_t* deg;
+    int32_t* offs;
+    unsigned long long* esum;
+};
+
+static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats) {
+    const int64_t n = A->n;
+    const DevCSR& out = A->csr;  // frontier advances along rows of A (vxm)
+    const DevCSR& in = A->csc;
+    hipStream_t st = stream();
+    void* p;
+    // slot 5: two queues + degrees + offsets (n+1 each) + counters
+    const size_t qbytes = sizeof(int32_t) * (size_t)(n + 1);
+    GBK_TRY(scratch(5, 4 * qbytes + 64, &p));
+    BfsScratch s;
+    s.q[0] = (int32_t*)p;
+    s.q[1] = s.q[0] + (n + 1);
+    s.deg = s.q[1] + (n + 1);
+    s.offs = s.deg + (n + 1);
+    s.cnt = s.offs + (n + 1);
+    s.esum = (unsigned long long*)(((uintptr_t)(s.cnt + 4) + 15) & ~(uintptr_t)15);
+    GBK_TRY(fill_value(GBK_INT32, level, n, 0.0));
+    hipLaunchKernelGGL(k_set_level, dim3(1), dim3(1), 0, st, level, source, s.q[0], s.cnt);
+    GBK_LAUNCH_CHECK();
+    int64_t nq = 1, m_u = A->nnz, levels = 0, pushes = 0, pulls = 0, traversed = 0;
+    int32_t cur = 1;
+    bool pull = false;
+    int qi = 0;
+    const double alpha = 14.0, beta = 24.0;
+    int G = 4;
+    {
+        double avg = n ? (double)in.nnz / (double)n : 0;
+        while (G < 32 && G < avg) G <<= 1;
+    }
+    while (nq > 0) {
+        // out-edges of the frontier
+        unsigned long long m_f = 0;
+        GBK_TRY(fill_value(GBK_INT64, s.esum, 1, 0.0));
+        hipLaunchKernelGGL(k_queue_edges, grid_n(nq), dim3(256), 0, st, out.row_ptr, s.q[qi], nq, s.esum);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipMemcpyAsync(&m_f, s.esum, sizeof(m_f), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        traversed += (int64_t)m_f;
+        if (!pull && (double)m_f > (double)m_u / alpha) pull = true;
+        else if (pull && (double)nq < (double)n / beta) pull = false;
+        m_u -= (int64_t)m_f;
+        int32_t* nxt = s.q[qi ^ 1];
+        int32_t* ncnt = s.cnt + (qi ^ 1);
+        GBK_TRY(fill_value(GBK_INT32, ncnt, 1, 0.0));
+        if (pull) {
+            ++pulls;
+#define GBK_PULL(GG)                                                                                  \
+    case GG:                                                                                          \
+        hipLaunchKernelGGL(k_bfs_pull<GG>, grid_n(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, level, cur, \
+                           nxt, ncnt);                                                                \
+        break;
+            switch (G) {
+                GBK_PULL(4)
+                GBK_PULL(8)
+                GBK_PULL(16)
+                GBK_PULL(32)
+            }
+#undef GBK_PULL
+            GBK_LAUNCH_CHECK();
+        } else {
+            ++pushes;
+            hipLaunchKernelGGL(k_frontier_deg, grid_n(nq), dim3(256), 0, st, out.row_ptr, s.q[qi], nq, s.deg);
+            GBK_LAUNCH_CHECK();
+            int32_t ne = 0;
+            GBK_TRY(exclusive_scan_i32(s.deg, s.offs, nq, &ne));
+            if (ne > 0) {
+                hipLaunchKernelGGL(k_bfs_push, grid_n(ne), dim3(256), 0, st, out.row_ptr, out.col, s.q[qi], nq,
+                                   s.offs, (int64_t)ne, level, cur + 1, nxt, ncnt);
+                GBK_LAUNCH_CHECK();
+            }
+        }
+        int32_t c = 0;
+        GBK_HIP(hipMemcpyAsync(&c, ncnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        nq = c;
+        qi ^= 1;
+        ++cur;
+        ++levels;
+    }
+    if (stats) {
+        stats[0] = levels;
+        stats[1] = pushes;
+        stats[2] = pulls;
+        stats[3] = traversed;
+    }
+    return GBK_SUCCESS;
+}
+
+// ------------------------------------------------------------------ SSSP --
+template <typename T> __device__ __forceinline__ bool atomic_min_improve(T* p, T v) {
+    if constexpr (std::is_same<T, int32_t>::value) {
+        return atomicMin(p, v) > v;
+    } else {
+        unsigned int* a = (unsigned int*)p;
+        unsigned int old = __atomic_load_n(a, __ATOMIC_RELAXED);
+        while (true) {
+            float o = __uint_as_float(old);
+            if (!(v < o)) return false;
+            unsigned int prev = atomicCAS(a, old, __float_as_uint(v));
+            if (prev == old) return true;
+            old = prev;
+        }
+    }
+}
+
+template <typename T> __device__ __forceinline__ T dist_add(T d, T w) {
+    if constexpr (std::is_same<T, int32_t>::value) return sat_add<int32_t>(d, w);
+    else return d + w;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sssp_relax(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const T* __restrict__ w, const int32_t* __restrict__ q, int64_t nq,
+                                                    const int32_t* __restrict__ offs, int64_t nedges,
+                                                    T* __restrict__ dist, int32_t* __restrict__ inq,
+                                                    int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count,
+                                                    int32_t round) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    int32_t v = 0;
+    if (e < nedges) {
+        int64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const int u = q[lo];
+        const int64_t k = rp[u] + (e - offs[lo]);
+        v = col[k];
+        const T du = dist[u];
+        const T nd = dist_add(du, w[k]);
+        if (nd < dist[v] && atomic_min_improve(dist + v, nd)) {
+            // enqueue v once per round
+            take = (atomicExch(inq + v, round) != round);
+        }
+    }
+    wave_append(take, v, nq_out, nq_count);
+}
+
+template <typename T> __global__ void k_sssp_init(T* dist, int64_t n, int64_t s, T inf, int32_t* q, int32_t* count,
+                                                  int32_t* inq) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        dist[i] = (i == s) ? T(0) : inf;
+        inq[i] = -1;
+    }
+    if (i == 0) {
+        q[0] = (int32_t)s;
+        *count = 1;
+    }
+}
+
+template <typename T> static int sssp_run(gbk_matrix A, int64_t source, T* dist_dev, int64_t* stats) {
+    const int64_t n = A->n;
+    const DevCSR& m = A->csr;
+    hipStream_t st = stream();
+    void* p;
+    const size_t qbytes = sizeof(int32_t) * (size_t)(n + 1);
+    GBK_TRY(scratch(5, 5 * qbytes + 64, &p));
+    int32_t* q[2] = {(int32_t*)p, (int32_t*)p + (n + 1)};
+    int32_t* deg = q[1] + (n + 1);
+    int32_t* offs = deg + (n + 1);
+    int32_t* inq = offs + (n + 1);
+    int32_t* cnt = inq + (n + 1);
+    const T inf = std::is_same<T, int32_t>::value ? (T)INT32_MAX : (T)FLT_MAX;
+    hipLaunchKernelGGL(k_sssp_init<T>, grid_n(n), dim3(256), 0, st, dist_dev, n, source, inf, q[0], cnt, inq);
+    GBK_LAUNCH_CHECK();
+    int64_t nq = 1, rounds = 0, relaxed = 0;
+    int qi = 0;
+    while (nq > 0) {
+        hipLaunchKernelGGL(k_frontier_deg, grid_n(nq), dim3(256), 0, st, m.row_ptr, q[qi], nq, deg);
+        GBK_LAUNCH_CHECK();
+        int32_t ne = 0;
+        GBK_TRY(exclusive_scan_i32(deg, offs, nq, &ne));
+        GBK_TRY(fill_value(GBK_INT32, cnt + (qi ^ 1), 1, 0.0));
+        if (ne > 0) {
+            hipLaunchKernelGGL((k_sssp_relax<T>), grid_n(ne), dim3(256), 0, st, m.row_ptr, m.col, (const T*)m.val,
+                               q[qi], nq, offs, (int64_t)ne, dist_dev, inq, q[qi ^ 1], cnt + (qi ^ 1), (int32_t)rounds);
+            GBK_LAUNCH_CHECK();
+        }
+        relaxed += ne;
+        int32_t c = 0;
+        GBK_HIP(hipMemcpyAsync(&c, cnt + (qi ^ 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        nq = c;
+        qi ^= 1;
+        ++rounds;
+    }
+    if (stats) {
+        stats[0] = rounds;
+        stats[1] = relaxed;
+    }
+    return GBK_SUCCESS;
+}
+
+// -------------------------------------------------------------------- CC --
+__global__ void k_cc_init(int32_t* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+// Hook: for every edge (u, v) with different parents, the larger parent is
+// lowered to the smaller one.
+__global__ __launch_bounds__(256) void k_cc_hook(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                 int64_t n, int32_t* __restrict__ parent, int32_t* __restrict__ changed) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t u = gid >> 2;
+    const int lane = (int)(gid & 3);
+    if (u >= n) return;
+    bool ch = false;
+    const int e1 = rp[u + 1];
+    for (int e = rp[u] + lane; e < e1; e += 4) {
+        const int v = col[e];
+        int pu = parent[u], pv = parent[v];
+        if (pu == pv) continue;
+        int hi = pu > pv ? pu : pv, lo = pu > pv ? pv : pu;
+        if (atomicMin(parent + hi, lo) > lo) ch = true;
+    }
+    if (ch) *changed = 1;
+}
+
+__global__ void k_cc_jump(int32_t* __restrict__ parent, int64_t n, int32_t* __restrict__ changed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t p = parent[i];
+    int32_t pp = parent[p];
+    if (pp != p) {
+        while (true) {
+            int32_t q = parent[pp];
+            if (q == pp) break;
+            pp = q;
+        }
+        parent[i] = pp;
+        *changed = 1;
+    }
+}
+
+static int cc_run(gbk_matrix A, int32_t* parent, int64_t* stats) {
+    const int64_t n = A->n;
+    const DevCSR& m = A->csr;
+    hipStream_t st = stream();
+    void* p;
+    GBK_TRY(scratch(6, 64, &p));
+    int32_t* changed = (int32_t*)p;
+    hipLaunchKernelGGL(k_cc_init, grid_n(n), dim3(256), 0, st, parent, n);
+    GBK_LAUNCH_CHECK();
+    int64_t iters = 0;
+    while (true) {
+        GBK_TRY(fill_value(GBK_INT32, changed, 1, 0.0));
+        hipLaunchKernelGGL(k_cc_hook, grid_n(n * 4), dim3(256), 0, st, m.row_ptr, m.col, n, parent, changed);
+        GBK_LAUNCH_CHECK();
+        // pointer jumping to a fixed point
+        hipLaunchKernelGGL(k_cc_jump, grid_n(n), dim3(256), 0, st, parent, n, changed);
+        GBK_LAUNCH_CHECK();
+        int32_t c = 0;
+        GBK_HIP(hipMemcpyAsync(&c, changed, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        ++iters;
+        if (!c) break;
+        if (iters > 4 * 64 + n) return fail(GBK_PANIC, "connected components did not converge");
+    }
+    if (stats) stats[0] = iters;
+    return GBK_SUCCESS;
+}
+
+// -------------------------------------------------------------------- PR --
+// w = t ./ d with d = max(deg / damping, 1 / damping); also sums the ranks of
+// sinks (deg == 0) into *sink.
+__global__ __launch_bounds__(256) void k_pr_prep(const int32_t* __restrict__ crp, const float* __restrict__ t,
+                                                 float* __restrict__ w, float* __restrict__ r, int64_t n, float damping,
+                                                 double* __restrict__ sink) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        const int deg = crp[i + 1] - crp[i];
+        float d = (float)deg / damping;
+        const float dmin = 1.0f / damping;
+        if (d < dmin) d = dmin;
+        const float ti = t[i];
+        w[i] = ti / d;
+        if (deg == 0) s = ti;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(sink, s);
+}
+
+__global__ void k_pr_fill(float* __restrict__ r, int64_t n, const double* __restrict__ sink, double base, double coef) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = (float)(base + coef * *sink);
+}
+
+__global__ __launch_bounds__(256) void k_pr_diff(const float* __restrict__ t, const float* __restrict__ r, int64_t n,
+                                                 double* __restrict__ diff) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n) s = fabs((double)t[i] - (double)r[i]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(diff, s);
+}
+
+static int pr_run(gbk_matrix A, float* ranks_dev, int itermax, double damping, double tol, int* iters_out) {
+    if (A->type != GBK_FP32) return fail(GBK_DOMAIN_MISMATCH, "PageRank needs an FP32 matrix");
+    const int64_t n = A->n;
+    hipStream_t st = stream();
+    float* t = (float*)A->ws.buf[0];
+    float* w = (float*)A->ws.buf[1];
+    void* p;
+    GBK_TRY(scratch(6, 64, &p));
+    double* acc = (double*)p;  // [0] sink sum, [1] L1 diff
+    float* r = ranks_dev;
+    GBK_TRY(fill_value(GBK_FP32, r, n, 1.0 / (double)n));
+    const DevCSR& csc = A->csc;
+    int it = 0;
+    for (it = 0; it < itermax;) {
+        // t = r (old ranks); w = t ./ d; sink sum
+        GBK_HIP(hipMemcpyAsync(t, r, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        GBK_TRY(fill_value(GBK_FP64, acc, 2, 0.0));
+        hipLaunchKernelGGL(k_pr_prep, grid_n(n), dim3(256), 0, st, csc.row_ptr, t, w, r, n, (float)damping, acc);
+        GBK_LAUNCH_CHECK();
+        // r = teleport; r += A (+.second) w
+        hipLaunchKernelGGL(k_pr_fill, grid_n(n), dim3(256), 0, st, r, n, acc, (1.0 - damping) / (double)n,
+                           damping / (double)n);
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(spmv_unmasked(A, 0, w, r, GBK_ADD_PLUS, GBK_MUL_SECOND, GBK_ACC_PLUS));
+        ++it;
+        if (tol > 0) {
+            hipLaunchKernelGGL(k_pr_diff, grid_n(n), dim3(256), 0, st, t, r, n, acc + 1);
+            GBK_LAUNCH_CHECK();
+            double d = 0;
+            GBK_HIP(hipMemcpyAsync(&d, acc + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+            GBK_HIP(hipStreamSynchronize(st));
+            if (d < tol) break;
+        }
+    }
+    GBK_HIP(hipStreamSynchronize(st));
+    if (iters_out) *iters_out = it;
+    return GBK_SUCCESS;
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+static int check_graph(gbk_matrix A) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    return ensure_init();
+}
+
+extern "C" int gbk_bfs_device(gbk_matrix A, int64_t source, int32_t* levels_dev, int64_t* stats) {
+    GBK_TRY(check_graph(A));
+    if (!levels_dev) return fail(GBK_NULL_POINTER, "null levels");
+    if (source < 0 || source >= A->n) return fail(GBK_INVALID_INDEX, "source %lld out of range", (long long)source);
+    return bfs_run(A, source, levels_dev, stats);
+}
+
+extern "C" int gbk_bfs(gbk_matrix A, int64_t source, int32_t* levels, int64_t* stats) {
+    GBK_TRY(check_graph(A));
+    if (!levels) return fail(GBK_NULL_POINTER, "null levels");
+    if (source < 0 || source >= A->n) return fail(GBK_INVALID_INDEX, "source %lld out of range", (long long)source);
+    int32_t* lv = (int32_t*)A->ws.buf[3];
+    GBK_TRY(bfs_run(A, source, lv, stats));
+    GBK_HIP(hipMemcpyAsync(levels, lv, sizeof(int32_t) * A->n, hipMemcpyDeviceToHost, stream()));
+    GBK_HIP(hipStreamSynchronize(stream()));
+    return GBK_SUCCESS;
+}
+
+extern "C" int gbk_sssp(gbk_matrix A, int64_t source, void* dist, int64_t* stats) {
+    GBK_TRY(check_graph(A));
+    if (!dist) return fail(GBK_NULL_POINTER, "null distances");
+    if (source < 0 || source >= A->n) return fail(GBK_INVALID_INDEX, "source %lld out of range", (long long)source);
+    void* d = A->ws.buf[3];
+    int rc;
+    if (A->type == GBK_INT32) rc = sssp_run<int32_t>(A, source, (int32_t*)d, stats);
+    else if (A->type == GBK_FP32) rc = sssp_run<float>(A, source, (float*)d, stats);
+    else return fail(GBK_DOMAIN_MISMATCH, "SSSP needs INT32 or FP32 weights");
+    GBK_TRY(rc);
+    GBK_HIP(hipMemcpyAsync(dist, d, 4 * A->n, hipMemcpyDeviceToHost, stream()));
+    GBK_HIP(hipStreamSynchronize(stream()));
+    return GBK_SUCCESS;
+}
+
+extern "C" int gbk_cc(gbk_matrix A, int32_t* labels, int64_t* stats) {
+    GBK_TRY(check_graph(A));
+    if (!labels) return fail(GBK_NULL_POINTER, "null labels");
+    int32_t* p = (int32_t*)A->ws.buf[3];
+    GBK_TRY(cc_run(A, p, stats));
+    GBK_HIP(hipMemcpyAsync(labels, p, sizeof(int32_t) * A->n, hipMemcpyDeviceToHost, stream()));
+    GBK_HIP(hipStreamSynchronize(stream()));
+    return GBK_SUCCESS;
+}
+
+extern "C" int gbk_pagerank(gbk_matrix A, float* ranks, int itermax, double damping, double tol, int* iters) {
+    GBK_TRY(check_graph(A));
+    if (!ranks) return fail(GBK_NULL_POINTER, "null ranks");
+    if (itermax < 0 || damping <= 0 || damping >= 1) return fail(GBK_INVALID_VALUE, "bad PageRank parameters");
+    float* r = (float*)A->ws.buf[2];
+    GBK_TRY(pr_run(A, r, itermax, damping, tol, iters));
+    GBK_HIP(hipMemcpyAsync(ranks, r, sizeof(float) * A->n, hipMemcpyDeviceToHost, stream()));
+    GBK_HIP(hipStreamSynchronize(stream()));
+    return GBK_SUCCESS;
+}

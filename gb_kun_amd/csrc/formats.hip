@@ -1,23 +1,426 @@
-// Optimized matrix representations (SELL-C-sigma, segmented CSR, LAV).
+// Optimized matrix representations (SELL-C-sigma, segmented CSR, LAV) on the
+// device, built from the CSR / CSC by format_build.cpp, one object per
+// orientation as the reference keeps data / transposed_data
+// (src/backend/matrix/build.hpp:6-118).
+//
+// SELL-C-sigma (spmv_sell_c.h:15-170): one lane per row, C = 64 (one wave per
+// chunk) or C = 32 (two chunks per wave); at step j the lanes of a chunk read
+// consecutive addresses chunk_ptr + j*C + r, so col/val stream coalesced.  A
+// lane folds its row in CSR order, the reference's left fold, so with FMA
+// contraction off the fp32 result equals the reference's CSR bits.  Rows of
+// "problematic" chunks run on the row-split CSR kernel.
+//
+// Segmented CSR (spmv_seg.h:9-308) and LAV (spmv_lav.h:8-156): per-segment
+// sums over a virtual CSR of (segment, row) pairs on the merge-path kernel,
+// with the tiles ordered so that a segment's tiles are dispatched to one XCD
+// and its slice of x stays in that XCD's L2; then a per-row merge of the
+// pair sums in segment order (deterministic, no atomics).  LAV first gathers
+// x into the degree-sorted column order.
+#include <algorithm>
+#include <vector>
+
 #include "gbk_objects.h"
+#include "host_formats.h"
 
 namespace gbk {
 
-int format_build(gbk_matrix A) {
-    return fail(GBK_INVALID_VALUE, "matrix storage format %d is not available in this build", A->format);
+int spmv_merge_mapped(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
+                      const int32_t* tile_map);
+int spmv_rows_list(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
+                   const int32_t* rows, int64_t count);
+
+struct SellC {
+    int C = 64, sigma = 512;
+    int64_t nchunks = 0, padded = 0, nproblem = 0;
+    int64_t* chunk_ptr = nullptr;
+    int32_t* chunk_len = nullptr;
+    int32_t* row_perm = nullptr;
+    int32_t* row_len = nullptr;
+    int32_t* col = nullptr;
+    void* val = nullptr;
+    int32_t* problem_rows = nullptr;
+};
+
+struct SegCSR {
+    int nseg = 0;
+    int64_t npairs = 0;
+    DevCSR v;  // virtual CSR: one row per (segment, row) pair
+    int32_t* tile_map = nullptr;
+    int32_t* rpp = nullptr;
+    int32_t* row_pairs = nullptr;
+    void* partial = nullptr;
+    int64_t n = 0;
+};
+
+struct Lav {
+    SegCSR seg;
+    int32_t* new_to_old = nullptr;
+    void* xperm = nullptr;
+    int nseg_dense = 0;
+};
+
+int build_merge_tiles(DevCSR& m, int type);
+
+template <typename V> static int upload(V** dst, const std::vector<V>& src) {
+    GBK_TRY(dev_alloc((void**)dst, sizeof(V) * (src.empty() ? 1 : src.size())));
+    if (!src.empty()) GBK_HIP(hipMemcpy(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+    return GBK_SUCCESS;
 }
 
-void format_free(gbk_matrix A) { (void)A; }
+static int download_csr(const DevCSR& m, int type, std::vector<int32_t>& rp, std::vector<int32_t>& col,
+                        std::vector<char>& val) {
+    const size_t vs = type_size(type);
+    rp.resize(m.nrows + 1);
+    col.resize(m.nnz);
+    val.resize(vs * m.nnz);
+    GBK_HIP(hipStreamSynchronize(stream()));
+    GBK_HIP(hipMemcpy(rp.data(), m.row_ptr, sizeof(int32_t) * (m.nrows + 1), hipMemcpyDeviceToHost));
+    if (m.nnz) {
+        GBK_HIP(hipMemcpy(col.data(), m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
+        GBK_HIP(hipMemcpy(val.data(), m.val, vs * m.nnz, hipMemcpyDeviceToHost));
+    }
+    return GBK_SUCCESS;
+}
+
+// ------------------------------------------------------------ SELL-C-sigma --
+static void free_sell(SellC* s) {
+    if (!s) return;
+    dev_free(s->chunk_ptr);
+    dev_free(s->chunk_len);
+    dev_free(s->row_perm);
+    dev_free(s->row_len);
+    dev_free(s->col);
+    dev_free(s->val);
+    dev_free(s->problem_rows);
+    delete s;
+}
+
+static int build_sell(const DevCSR& m, int type, int C, int sigma, SellC** out) {
+    std::vector<int32_t> rp, col;
+    std::vector<char> val;
+    GBK_TRY(download_csr(m, type, rp, col, val));
+    gbk_host::SellCHost h;
+    gbk_host::build_sell_c(m.nrows, rp.data(), col.data(), val.data(), type_size(type), C, sigma, h);
+    SellC* s = new SellC();
+    s->C = C;
+    s->sigma = sigma;
+    s->nchunks = h.nchunks;
+    s->padded = h.padded;
+    s->nproblem = (int64_t)h.problem_rows.size();
+    int rc = upload(&s->chunk_ptr, h.chunk_ptr);
+    if (rc == GBK_SUCCESS) rc = upload(&s->chunk_len, h.chunk_len);
+    if (rc == GBK_SUCCESS) rc = upload(&s->row_perm, h.row_perm);
+    if (rc == GBK_SUCCESS) rc = upload(&s->row_len, h.row_len);
+    if (rc == GBK_SUCCESS) rc = upload(&s->col, h.col);
+    if (rc == GBK_SUCCESS) rc = upload((char**)&s->val, h.val);
+    if (rc == GBK_SUCCESS) rc = upload(&s->problem_rows, h.problem_rows);
+    if (rc != GBK_SUCCESS) {
+        free_sell(s);
+        return rc;
+    }
+    *out = s;
+    return GBK_SUCCESS;
+}
+
+template <typename T, int ADD, int MUL>
+__global__ __launch_bounds__(256) void k_sell_c(const int64_t* __restrict__ chunk_ptr, const int32_t* __restrict__ row_perm,
+                                                const int32_t* __restrict__ row_len, const int32_t* __restrict__ col,
+                                                const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+                                                int64_t nchunks, int C, int accum) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = gid / C;
+    const int r = (int)(gid % C);
+    if (c >= nchunks) return;
+    const int64_t p = c * C + r;
+    const int32_t row = row_perm[p];
+    if (row < 0) return;
+    const int32_t L = row_len[p];
+    const int64_t base = chunk_ptr[c] + r;
+    T acc = add_identity<ADD, T>();
+    for (int32_t j = 0; j < L; ++j) {
+        const int64_t e = base + (int64_t)j * C;
+        acc = add_op<ADD, T>(acc, mul_op<MUL, T>(__builtin_nontemporal_load(val + e), x[__builtin_nontemporal_load(col + e)]));
+    }
+    y[row] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[row], acc);
+}
+
+static int sell_spmv(const SellC* s, const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum) {
+    GBK_TRY(dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
+            const int64_t threads = s->nchunks * s->C;
+            if (threads > 0) {
+                hipLaunchKernelGGL((k_sell_c<T, ADD, MUL>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                                   stream(), s->chunk_ptr, s->row_perm, s->row_len, s->col, (const T*)s->val,
+                                   (const T*)x, (T*)y, s->nchunks, s->C, accum);
+                GBK_LAUNCH_CHECK();
+            }
+            return GBK_SUCCESS;
+        });
+    }));
+    if (s->nproblem > 0) GBK_TRY(spmv_rows_list(m, type, x, y, add, mul, accum, s->problem_rows, s->nproblem));
+    return GBK_SUCCESS;
+}
+
+// --------------------------------------------------------- segmented CSR --
+static void free_seg_fields(SegCSR& s) {
+    dev_free(s.v.row_ptr);
+    dev_free(s.v.col);
+    dev_free(s.v.val);
+    dev_free(s.v.tiles);
+    dev_free(s.v.carry);
+    dev_free(s.tile_map);
+    dev_free(s.rpp);
+    dev_free(s.row_pairs);
+    dev_free(s.partial);
+    s = SegCSR();
+}
+
+// Tile order for the XCD-aware launch: block b runs on XCD b % 8 and gets a
+// tile of a segment assigned to that XCD (segment s -> XCD s % 8) while any
+// remain, else a tile of the group with the most left.
+static std::vector<int32_t> xcd_tile_map(const std::vector<int2>& tiles, const std::vector<int64_t>& seg_pair_start,
+                                         int nseg) {
+    const int64_t nt = (int64_t)tiles.size() - 1;
+    const int G = 8;
+    std::vector<std::vector<int32_t>> grp(G);
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t row = tiles[t].x;
+        int s = (int)(std::upper_bound(seg_pair_start.begin(), seg_pair_start.end(), row) - seg_pair_start.begin()) - 1;
+        s = std::max(0, std::min(s, nseg - 1));
+        grp[s % G].push_back((int32_t)t);
+    }
+    std::vector<size_t> at(G, 0);
+    std::vector<int32_t> map;
+    map.reserve(nt);
+    for (int64_t b = 0; b < nt; ++b) {
+        int g = (int)(b % G);
+        if (at[g] >= grp[g].size()) {
+            size_t best = 0;
+            for (int h = 0; h < G; ++h)
+                if (grp[h].size() - at[h] > best) {
+                    best = grp[h].size() - at[h];
+                    g = h;
+                }
+        }
+        map.push_back(grp[g][at[g]++]);
+    }
+    return map;
+}
+
+static int build_seg_common(const gbk_host::SegHost& h, int type, int64_t n, SegCSR& s) {
+    const size_t vs = type_size(type);
+    s.nseg = h.nseg;
+    s.npairs = h.npairs;
+    s.n = n;
+    s.v.nrows = s.v.ncols = h.npairs;
+    s.v.nnz = (int64_t)h.col.size();
+    GBK_TRY(upload(&s.v.row_ptr, h.vrp));
+    GBK_TRY(upload(&s.v.col, h.col));
+    GBK_TRY(upload((char**)&s.v.val, h.val));
+    GBK_TRY(build_merge_tiles(s.v, type));
+    std::vector<int2> tiles(s.v.ntiles + 1);
+    GBK_HIP(hipMemcpy(tiles.data(), s.v.tiles, sizeof(int2) * (s.v.ntiles + 1), hipMemcpyDeviceToHost));
+    GBK_TRY(upload(&s.tile_map, xcd_tile_map(tiles, h.seg_pair_start, h.nseg)));
+    GBK_TRY(upload(&s.rpp, h.rpp));
+    GBK_TRY(upload(&s.row_pairs, h.row_pairs));
+    GBK_TRY(dev_alloc(&s.partial, vs * (h.npairs > 0 ? h.npairs : 1)));
+    (void)vs;
+    return GBK_SUCCESS;
+}
+
+static int64_t seg_cols_default(int64_t n) {
+    // 2 MiB of fp32 x per segment: half of one XCD's 4 MiB L2
+    (void)n;
+    return 512 * 1024;
+}
+
+static int build_seg(const DevCSR& m, int type, SegCSR** out) {
+    std::vector<int32_t> rp, col;
+    std::vector<char> val;
+    GBK_TRY(download_csr(m, type, rp, col, val));
+    const int64_t n = m.nrows;
+    std::vector<int64_t> bounds;
+    const int64_t sc = seg_cols_default(n);
+    for (int64_t b = 0; b < n; b += sc) bounds.push_back(b);
+    bounds.push_back(n);
+    if (bounds.size() < 2) bounds = {0, n};
+    gbk_host::SegHost h;
+    gbk_host::build_segmented(n, rp.data(), col.data(), val.data(), type_size(type), bounds, nullptr, h);
+    SegCSR* s = new SegCSR();
+    int rc = build_seg_common(h, type, n, *s);
+    if (rc != GBK_SUCCESS) {
+        free_seg_fields(*s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return GBK_SUCCESS;
+}
+
+template <typename T, int ADD>
+__global__ void k_seg_merge(const int32_t* __restrict__ rpp, const int32_t* __restrict__ row_pairs,
+                            const T* __restrict__ partial, T* __restrict__ y, int64_t n, int accum) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    T acc = add_identity<ADD, T>();
+    for (int p = rpp[r]; p < rpp[r + 1]; ++p) acc = add_op<ADD, T>(acc, partial[row_pairs[p]]);
+    y[r] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[r], acc);
+}
+
+static int seg_spmv(const SegCSR* s, int type, const void* x, void* y, int add, int mul, int accum) {
+    GBK_TRY(spmv_merge_mapped(s->v, type, x, s->partial, add, mul, GBK_ACC_NONE, s->tile_map));
+    return dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value;
+            (void)b;
+            if (s->n > 0) {
+                hipLaunchKernelGGL((k_seg_merge<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0,
+                                   stream(), s->rpp, s->row_pairs, (const T*)s->partial, (T*)y, s->n, accum);
+                GBK_LAUNCH_CHECK();
+            }
+            return GBK_SUCCESS;
+        });
+    });
+}
+
+// ------------------------------------------------------------------- LAV --
+static int build_lav(const DevCSR& m, int type, Lav** out) {
+    std::vector<int32_t> rp, col;
+    std::vector<char> val;
+    GBK_TRY(download_csr(m, type, rp, col, val));
+    const int64_t n = m.nrows;
+    std::vector<int32_t> o2n, n2o;
+    std::vector<int64_t> bounds;
+    gbk_host::lav_relabel(n, rp.data(), col.data(), 0.8, seg_cols_default(n), o2n, n2o, bounds);
+    gbk_host::SegHost h;
+    gbk_host::build_segmented(n, rp.data(), col.data(), val.data(), type_size(type), bounds, o2n.data(), h);
+    Lav* l = new Lav();
+    int rc = build_seg_common(h, type, n, l->seg);
+    if (rc == GBK_SUCCESS) rc = upload(&l->new_to_old, n2o);
+    if (rc == GBK_SUCCESS) rc = dev_alloc(&l->xperm, type_size(type) * (n > 0 ? n : 1));
+    if (rc != GBK_SUCCESS) {
+        free_seg_fields(l->seg);
+        dev_free(l->new_to_old);
+        dev_free(l->xperm);
+        delete l;
+        return rc;
+    }
+    l->nseg_dense = (int)bounds.size() - 1;
+    *out = l;
+    return GBK_SUCCESS;
+}
+
+template <typename T>
+__global__ void k_gather(const int32_t* __restrict__ idx, const T* __restrict__ x, T* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x[idx[i]];
+}
+
+static int lav_spmv(const Lav* l, int type, const void* x, void* y, int add, int mul, int accum) {
+    const int64_t n = l->seg.n;
+    if (n > 0) {
+        GBK_TRY(dispatch_type(type, [&](auto tv) -> int {
+            typedef decltype(tv) T;
+            hipLaunchKernelGGL(k_gather<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream(), l->new_to_old,
+                               (const T*)x, (T*)l->xperm, n);
+            GBK_LAUNCH_CHECK();
+            return GBK_SUCCESS;
+        }));
+    }
+    return seg_spmv(&l->seg, type, l->xperm, y, add, mul, accum);
+}
+
+// ------------------------------------------------------------ interface --
+void format_free(gbk_matrix A) {
+    for (int t = 0; t < 2; ++t) {
+        free_sell(A->sell[t]);
+        A->sell[t] = nullptr;
+        if (A->seg[t]) {
+            free_seg_fields(*A->seg[t]);
+            delete A->seg[t];
+            A->seg[t] = nullptr;
+        }
+        if (A->lav[t]) {
+            free_seg_fields(A->lav[t]->seg);
+            dev_free(A->lav[t]->new_to_old);
+            dev_free(A->lav[t]->xperm);
+            delete A->lav[t];
+            A->lav[t] = nullptr;
+        }
+    }
+}
+
+int format_build(gbk_matrix A) {
+    GBK_TRY(ensure_init());
+    format_free(A);
+    for (int t = 0; t < 2; ++t) {
+        const DevCSR& m = t ? A->csc : A->csr;
+        int rc = GBK_SUCCESS;
+        if (A->format == GBK_FMT_SELL_C) rc = build_sell(m, A->type, A->sell_c, A->sigma, &A->sell[t]);
+        else if (A->format == GBK_FMT_CSR_SEG) rc = build_seg(m, A->type, &A->seg[t]);
+        else if (A->format == GBK_FMT_LAV) rc = build_lav(m, A->type, &A->lav[t]);
+        if (rc != GBK_SUCCESS) {
+            format_free(A);
+            return rc;
+        }
+    }
+    return GBK_SUCCESS;
+}
 
 int format_spmv(gbk_matrix A, int transposed, const void* x, void* y, int add, int mul, int accum, bool* handled) {
-    (void)A; (void)transposed; (void)x; (void)y; (void)add; (void)mul; (void)accum;
     *handled = false;
+    const int t = transposed ? 1 : 0;
+    const DevCSR& m = t ? A->csc : A->csr;
+    if (A->format == GBK_FMT_SELL_C && A->sell[t]) {
+        *handled = true;
+        return sell_spmv(A->sell[t], m, A->type, x, y, add, mul, accum);
+    }
+    if (A->format == GBK_FMT_CSR_SEG && A->seg[t]) {
+        *handled = true;
+        return seg_spmv(A->seg[t], A->type, x, y, add, mul, accum);
+    }
+    if (A->format == GBK_FMT_LAV && A->lav[t]) {
+        if (transposed) return fail(GBK_INVALID_VALUE, "unsupported matrix storage format in VSpM");
+        *handled = true;
+        return lav_spmv(A->lav[t], A->type, x, y, add, mul, accum);
+    }
     return GBK_SUCCESS;
 }
 
 int64_t format_bytes(gbk_matrix A) {
-    (void)A;
-    return 0;
+    const size_t vs = type_size(A->type);
+    int64_t b = 0;
+    for (int t = 0; t < 2; ++t) {
+        if (const SellC* s = A->sell[t])
+            b += (int64_t)(s->padded * (4 + vs) + s->nchunks * (8 + 4 + 8 * s->C) + s->nproblem * 4);
+        const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
+        if (sg) b += (int64_t)(sg->npairs * (4 + 4 + vs + 4) + sg->v.nnz * (4 + vs) + sg->n * 4 + sg->v.ntiles * 28);
+        if (A->lav[t]) b += A->lav[t]->seg.n * (4 + vs);
+    }
+    return b;
 }
 
 }  // namespace gbk
+
+using namespace gbk;
+
+// Format statistics for the autotuner and reports: [0] padded SELL-C entries,
+// [1] problematic SELL-C rows, [2] segments, [3] (segment, row) pairs.
+extern "C" int gbk_matrix_format_stats(gbk_matrix A, int transposed, int64_t* stats) {
+    if (!A || !stats) return fail(GBK_NULL_POINTER, "null argument");
+    const int t = transposed ? 1 : 0;
+    for (int i = 0; i < 4; ++i) stats[i] = 0;
+    if (A->sell[t]) {
+        stats[0] = A->sell[t]->padded;
+        stats[1] = A->sell[t]->nproblem;
+    }
+    const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
+    if (sg) {
+        stats[2] = sg->nseg;
+        stats[3] = sg->npairs;
+    }
+    return GBK_SUCCESS;
+}

@@ -25,6 +25,12 @@
 
 #define GBK_HD __host__ __device__ __forceinline__
 
+// add(acc, mul(a, x)) is two roundings in the reference (a separate multiply
+// and add); keep it so on the device instead of fusing it into an FMA, so a
+// kernel that folds a row in storage order reproduces the reference's fp32
+// bits exactly.
+#pragma clang fp contract(off)
+
 namespace gbk {
 
 // ---------------------------------------------------------------- errors --
@@ -92,6 +98,11 @@ inline bool valid_type(int t) { return t >= GBK_FP32 && t <= GBK_BOOL; }
 
 template <typename T> GBK_HD T tbool(bool b) { return b ? T(1) : T(0); }
 template <typename T> GBK_HD bool truth(T v) { return v != T(0); }
+// C++ conversion U -> W, with bool targets taking (u != 0) as a C++ bool would.
+template <typename W, typename U> GBK_HD W cvt(U u) {
+    if constexpr (std::is_same<W, uint8_t>::value && !std::is_same<U, uint8_t>::value) return u != U(0) ? 1 : 0;
+    else return (W)u;
+}
 
 template <typename T> constexpr bool is_int_v = std::is_integral<T>::value && !std::is_same<T, boolean>::value;
 template <typename T> constexpr bool is_bool_v = std::is_same<T, boolean>::value;

@@ -88,14 +88,16 @@ __global__ __launch_bounds__(MP_THREADS) void k_spmv_merge(const int32_t* __rest
                                                            const T* __restrict__ val, const T* __restrict__ x,
                                                            T* __restrict__ y, int32_t nrows,
                                                            const int2* __restrict__ tiles, T* __restrict__ carry,
-                                                           int accum) {
+                                                           int accum, const int32_t* __restrict__ tile_map) {
     __shared__ int32_t s_end[MP_TILE + 1];
     __shared__ T s_prod[MP_TILE];
     __shared__ T s_y[MP_TILE];
     __shared__ T s_wv[MP_THREADS / 64];
     __shared__ int s_wf[MP_THREADS / 64];
 
-    const int tile = blockIdx.x;
+    // tile_map reorders tiles across the dispatch order (workgroups go to the
+    // XCDs round-robin), e.g. to keep a column segment on one XCD's L2
+    const int tile = tile_map ? tile_map[blockIdx.x] : (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
     const int2 c0 = tiles[tile], c1 = tiles[tile + 1];
@@ -298,11 +300,11 @@ static int launch_rows(const DevCSR& m, const T* x, T* y, int64_t count, const i
 }
 
 template <typename T, int ADD, int MUL>
-static int launch_merge(const DevCSR& m, const T* x, T* y, int accum) {
+static int launch_merge(const DevCSR& m, const T* x, T* y, int accum, const int32_t* tile_map = nullptr) {
     if (m.ntiles == 0) return GBK_SUCCESS;
     hipStream_t st = stream();
     hipLaunchKernelGGL((k_spmv_merge<T, ADD, MUL>), dim3((unsigned)m.ntiles), dim3(MP_THREADS), 0, st, m.row_ptr, m.col,
-                       (const T*)m.val, x, y, (int32_t)m.nrows, m.tiles, (T*)m.carry, accum);
+                       (const T*)m.val, x, y, (int32_t)m.nrows, m.tiles, (T*)m.carry, accum, tile_map);
     GBK_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_spmv_merge_fixup<T, ADD>), dim3((unsigned)((m.ntiles + 255) / 256)), dim3(256), 0, st,
                        m.row_ptr, m.tiles, m.ntiles, (const T*)m.carry, y, accum);
@@ -318,6 +320,34 @@ int spmv_csr_unmasked(const DevCSR& m, int type, const void* x, void* y, int add
             if (order_sensitive(ADD))
                 return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, m.nrows, nullptr, nullptr, 0, 1, accum, 1);
             return launch_merge<T, ADD, MUL>(m, (const T*)x, (T*)y, accum);
+        });
+    });
+}
+
+// Merge-path SpMV of a CSR with an optional tile order (used by the segmented formats).
+int spmv_merge_mapped(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
+                      const int32_t* tile_map) {
+    return dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
+            if (order_sensitive(ADD))
+                return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, m.nrows, nullptr, nullptr, 0, 1, accum, 1);
+            return launch_merge<T, ADD, MUL>(m, (const T*)x, (T*)y, accum, tile_map);
+        });
+    });
+}
+
+// Row-group SpMV over a list of rows of a CSR (G = 64 lanes per row unless the
+// monoid is order-sensitive).
+int spmv_rows_list(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
+                   const int32_t* rows, int64_t count) {
+    return dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
+            return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, count, rows, nullptr, 0, 1, accum,
+                                            order_sensitive(ADD) ? 1 : 64);
         });
     });
 }

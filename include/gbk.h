@@ -213,6 +213,11 @@ int gbk_matrix_device_csr(gbk_matrix A, int transposed, const int32_t** row_ptr,
                           const int32_t** col_ids, const void** vals);
 /* Bytes of HBM held by the matrix (all formats and workspaces). */
 int gbk_matrix_device_bytes(gbk_matrix A, int64_t* bytes);
+/* Statistics of the optimized representation of one orientation (the
+ * reference's MatrixSellC::print_stats / segment counts): stats[0] padded
+ * SELL-C entries, [1] rows of problematic SELL-C chunks, [2] segments,
+ * [3] (segment, row) pairs. */
+int gbk_matrix_format_stats(gbk_matrix A, int transposed, int64_t* stats);
 
 /* ------------------------------------------------------------------------ */
 /* Vector — replaces lablas::Vector<T> (src/cpp_graphblas/vector.hpp:11-123)

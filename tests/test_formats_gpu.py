@@ -1,0 +1,93 @@
+"""GPU parity of the optimized formats (SELL-C-sigma, segmented CSR, LAV).
+
+Every format must give the CSR result: bit-exact for integer semirings; for
+fp32 plus.times SELL-C folds each row in CSR order, so it must reproduce the
+reference's CSR bits (oracle left fold); segmented CSR and LAV are within
+1e-5 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from tests.graphs import special_graphs
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = [("sell64", 6, 64, 512), ("sell32", 6, 32, 512), ("sell64_s1", 6, 64, 1), ("seg", 3, 0, 0),
+           ("lav", 4, 0, 0)]
+
+
+def _graphs(gbk):
+    gs = {"rmat12": gbk.gen_rmat(12, 16, seed=31), "er12": gbk.gen_uniform(12, 8, seed=32),
+          "rmat16": gbk.gen_rmat(16, 16, seed=33)}
+    gs.update(special_graphs())
+    return gs
+
+
+@pytest.mark.parametrize("name,fmt,C,sigma", FORMATS)
+def test_format_int_exact(gbk, oracle, name, fmt, C, sigma):
+    for gname, (rp, col) in _graphs(gbk).items():
+        vals = (1 + (gbk.uniform_f32(41, len(col)) * 255).astype(np.int64)).astype(np.int32)
+        A = gbk.Matrix.from_csr(rp, col, vals, fmt=fmt, sell_c=C, sigma=sigma)
+        n = A.n
+        x = (gbk.uniform_f32(42, n) * 1000).astype(np.int32)
+        for sr in ("PlusMultiplies", "MinimumPlus"):
+            add, mul = gbk.SEMIRINGS[sr]
+            w = gbk.Vector(n, gbk.INT32)
+            gbk.mxv(w, None, gbk.ACC_NONE, sr, A, gbk.Vector.dense(x))
+            rp2 = np.concatenate([rp, np.full(n + 1 - len(rp), rp[-1])])
+            want = oracle.spmv(rp2, col, vals, x, np.zeros(n, np.int32), add, mul)
+            assert np.array_equal(w.to_numpy(), want), f"{name} {gname} {sr}"
+        if fmt != 4:  # vxm through the transposed representation (LAV has none, as in the reference)
+            w = gbk.Vector(n, gbk.INT32)
+            gbk.vxm(w, None, gbk.ACC_NONE, "PlusMultiplies", gbk.Vector.dense(x), A)
+            trp, tcol, tval = A.export(transposed=True)
+            want = oracle.spmv(trp, tcol, tval, x, np.zeros(n, np.int32), 0, 0)
+            assert np.array_equal(w.to_numpy(), want), f"{name} {gname} vxm"
+
+
+@pytest.mark.parametrize("name,fmt,C,sigma", FORMATS)
+def test_format_fp32(gbk, oracle, name, fmt, C, sigma):
+    for gname, (rp, col) in _graphs(gbk).items():
+        vals = gbk.uniform_f32(51, len(col))
+        A = gbk.Matrix.from_csr(rp, col, vals, fmt=fmt, sell_c=C, sigma=sigma)
+        n = A.n
+        x = gbk.uniform_f32(52, n)
+        w0 = gbk.uniform_f32(53, n)
+        w = gbk.Vector.dense(w0)
+        gbk.mxv(w, None, gbk.ACC_PLUS, "PlusMultiplies", A, gbk.Vector.dense(x))
+        got = w.to_numpy()
+        rp2 = np.concatenate([rp, np.full(n + 1 - len(rp), rp[-1])])
+        if fmt == 6:
+            want = oracle.spmv(rp2, col, vals, x, w0, 0, 0, accum=2)
+            assert np.array_equal(got, want), f"{name} {gname}: SELL-C must fold rows in CSR order"
+        else:
+            y64, absum = oracle.spmv_f64(rp2, col, vals, x)
+            err = np.abs(got.astype(np.float64) - (w0.astype(np.float64) + y64))
+            assert np.all(err <= 1e-5 * (absum + np.abs(w0)) + 1e-30), f"{name} {gname}"
+
+
+def test_sell_c_matches_reference_csr_bits(gbk):
+    """Golden: the compiled reference's fp32 CSR SpMV, reproduced bit for bit by SELL-C."""
+    import os
+    d = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "spmv_rmat9_f32.npz")))
+    A = gbk.Matrix.from_csr(d["rp"], d["col"], d["vals"], fmt=6, sell_c=64, sigma=512)
+    w = gbk.Vector(A.n, gbk.FP32)
+    gbk.mxv(w, None, gbk.ACC_NONE, "PlusMultiplies", A, gbk.Vector.dense(d["x"]))
+    assert np.array_equal(w.to_numpy(), d["y_plus_times_csr"])
+
+
+def test_problematic_chunks_not_identity(gbk, oracle):
+    """A chunk padded 2x with > 1024 entries goes to the row kernel; its rows
+    get the real fold (the reference returns the identity, spmv_sell_c.h:133-138)."""
+    from tests.graphs import csr_from_edges
+    n = 256
+    # five full rows (1280 entries) in a 64-row chunk whose other rows hold one entry
+    edges = [(r, c) for r in range(5) for c in range(n)] + [(r, (r * 7) % n) for r in range(5, 64)]
+    rp, col = csr_from_edges(n, edges)
+    vals = np.ones(len(col), np.int32)
+    A = gbk.Matrix.from_csr(rp, col, vals, fmt=6, sell_c=64, sigma=512)
+    assert A.format_stats()[1] > 0, "expected a problematic chunk"
+    x = np.arange(n, dtype=np.int32)
+    w = gbk.Vector(n, gbk.INT32)
+    gbk.mxv(w, None, gbk.ACC_NONE, "PlusMultiplies", A, gbk.Vector.dense(x))
+    assert np.array_equal(w.to_numpy(), oracle.spmv(rp, col, vals, x, np.zeros(n, np.int32), 0, 0))

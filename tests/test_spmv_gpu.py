@@ -52,7 +52,8 @@ def _run_mxv(gbk, A, u_np, w_np, semiring, accum=0, mask=None, desc_mask=6, vxm=
         gbk.vxm(w, m, accum, semiring, u, A, desc_mask)
     else:
         gbk.mxv(w, m, accum, semiring, A, u, desc_mask)
-    return w.to_numpy(), w.storage
+    storage = w.storage  # read first: to_numpy() makes the vector dense
+    return w.to_numpy(), storage
 
 
 EXACT_CASES = [
