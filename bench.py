@@ -43,6 +43,9 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--format", default="csr", choices=["csr", "sell_c", "csr_seg", "lav"])
+    ap.add_argument("--graph", default="rmat", choices=["rmat", "er"])
+    ap.add_argument("--sell-c", type=int, default=0)
+    ap.add_argument("--sigma", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
@@ -56,25 +59,30 @@ def main():
     import gb_kun_amd as g
 
     t0 = time.time()
-    rp, col = g.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1, symmetric=False, permute=True)
+    if args.graph == "rmat":
+        rp, col = g.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1, symmetric=False, permute=True)
+    else:
+        rp, col = g.gen_uniform(args.scale, args.edge_factor, seed=1)
     n = len(rp) - 1
     nnz = len(col)
     vals = g.inv_coldeg_f32(n, col)
     x_np = g.uniform_f32(3, n)
-    log(f"graph: R-MAT scale {args.scale} ef {args.edge_factor}: n={n} nnz={nnz} "
+    log(f"graph: {args.graph} scale {args.scale} ef {args.edge_factor}: n={n} nnz={nnz} "
         f"empty rows={int(np.count_nonzero(np.diff(rp) == 0))} max row={int(np.diff(rp).max())} "
         f"({time.time() - t0:.1f} s)")
 
     fmt = {"csr": g.FMT_CSR, "sell_c": g.FMT_SELL_C, "csr_seg": g.FMT_CSR_SEG, "lav": g.FMT_LAV}[args.format]
     t0 = time.time()
-    A = g.Matrix.from_csr(rp, col, vals, fmt=fmt)
+    A = g.Matrix.from_csr(rp, col, vals, fmt=fmt, sell_c=args.sell_c, sigma=args.sigma)
     log(f"device matrix built in {time.time() - t0:.1f} s, {A.device_bytes() / 2**20:.0f} MiB of HBM")
     cu, arch = g.device_info()
 
-    stream = torch.cuda.current_stream()
-    g.check(g.lib.gbk_set_stream(g._vp(stream.cuda_stream)))
+    # The kernels run on the library's own stream; the timing events are
+    # recorded on that same stream.
+    stream = torch.cuda.ExternalStream(int(g.lib.gbk_get_stream()))
     x = torch.from_numpy(x_np).cuda()
     y = torch.zeros(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
 
     def step():
         g.spmv_device(A, x.data_ptr(), y.data_ptr())
@@ -92,6 +100,7 @@ def main():
     log(f"spot check vs FP64 on {len(rows)} rows: max rel err {err:.2e}")
 
     torch.cuda.synchronize()
+    g.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     w0 = time.perf_counter()
@@ -99,6 +108,7 @@ def main():
     for _ in range(args.steps):
         step()
     ev1.record(stream)
+    ev1.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     ms = ev0.elapsed_time(ev1) / args.steps
@@ -133,7 +143,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded R-MAT, values 1/coldeg)",
-        "config": {"workload": f"plus.times fp32 SpMV, R-MAT scale {args.scale}, edge factor {args.edge_factor}, "
+        "config": {"workload": f"plus.times fp32 SpMV, {'R-MAT' if args.graph == 'rmat' else 'Erdos-Renyi'} scale "
+                               f"{args.scale}, edge factor {args.edge_factor}, "
                                f"{args.format} format", "n": n, "nnz": nnz, "format": args.format,
                    "parallelism": "single GPU"},
         "gteps": round(gteps, 3),

@@ -130,6 +130,12 @@ _sig("gbk_bfs_device", C.c_int, _vp, _i64, _vp, _pi64)
 _sig("gbk_sssp", C.c_int, _vp, _i64, _vp, _pi64)
 _sig("gbk_cc", C.c_int, _vp, _pi32, _pi64)
 _sig("gbk_pagerank", C.c_int, _vp, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
+_sig("gbk_spmm_device", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, Semiring, C.c_int)
+_sig("gbk_msbfs", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
+_sig("gbk_msbfs_device", C.c_int, _vp, _pi64, C.c_int, _vp, _pi64)
+_sig("gbk_ppr_multi", C.c_int, _vp, _pi64, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double,
+     C.POINTER(C.c_int))
+_sig("gbk_sssp_multi", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
 _sig("gbk_gen_rmat", C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
      C.c_int, _pi64, _pi64, C.POINTER(_pi64), C.POINTER(_pi32))
 _sig("gbk_gen_uniform", C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _pi64, _pi64, C.POINTER(_pi64),
@@ -429,6 +435,44 @@ def pagerank(A, itermax=100, damping=0.85, tol=0.0):
     it = C.c_int(0)
     check(lib.gbk_pagerank(A.h, _ptr(r, C.c_float), itermax, damping, tol, C.byref(it)))
     return r, it.value
+
+
+# ------------------------------------------------------------- batched ops
+def spmm_device(A, X_ptr, Y_ptr, k, semiring=(ADD_PLUS, MUL_TIMES), accum=ACC_NONE, transposed=False):
+    """Y = accum(Y, A (+.*) X) for k right-hand sides stored row-major (n x k) on the device."""
+    add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
+    check(lib.gbk_spmm_device(A.h, int(transposed), _vp(X_ptr), _vp(Y_ptr), int(k), Semiring(add, mul), accum))
+
+
+def msbfs(A, sources):
+    """Multi-source BFS (<= 64 sources): int32 levels[len(sources), n] (1-based, 0 unreached) and stats
+    (levels, push steps, pull steps, frontier edges, pairs reached)."""
+    src = np.ascontiguousarray(sources, dtype=np.int64)
+    lv = np.empty((len(src), A.n), dtype=np.int32)
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_msbfs(A.h, _ptr(src, C.c_int64), len(src), _ptr(lv, C.c_int32), _ptr(st, C.c_int64)))
+    return lv, st[:5]
+
+
+def ppr_multi(A, sources, itermax=100, damping=0.85, tol=1e-6):
+    """Personalized PageRank for <= 64 sources on an FP32 transition matrix: ranks[len(sources), n] and the
+    iterations each column ran."""
+    src = np.ascontiguousarray(sources, dtype=np.int64)
+    r = np.empty((len(src), A.n), dtype=np.float32)
+    it = np.zeros(len(src), dtype=np.int32)
+    check(lib.gbk_ppr_multi(A.h, _ptr(src, C.c_int64), len(src), _ptr(r, C.c_float), itermax, damping, tol,
+                            _ptr(it, C.c_int)))
+    return r, it
+
+
+def sssp_multi(A, sources):
+    """min.plus SSSP for <= 64 sources on an INT32 weight matrix: dist[len(sources), n] (INT32_MAX unreached)
+    and stats (rounds, relaxations)."""
+    src = np.ascontiguousarray(sources, dtype=np.int64)
+    d = np.empty((len(src), A.n), dtype=np.int32)
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_sssp_multi(A.h, _ptr(src, C.c_int64), len(src), _ptr(d, C.c_int32), _ptr(st, C.c_int64)))
+    return d, st[:2]
 
 
 # ------------------------------------------------------- element-wise ops

@@ -1,0 +1,41 @@
+// Internal interface of the batched kernels (batch.hip) and of the query
+// server built on them (serve.cpp).  Not part of the C ABI.
+#pragma once
+
+#include "gbk_objects.h"
+
+namespace gbk {
+
+// merge-path items (rows + nonzeros) walked by one lane group of the SpMM
+constexpr int SPMM_TILE = 256;
+
+// scratch slots owned by the batched entry points
+enum {
+    SLOT_SPMM_CARRY = 8,
+    SLOT_MSBFS = 9,
+    SLOT_SOURCES = 10,
+    SLOT_MSBFS_OUT = 11,
+    SLOT_PPR = 12,
+    SLOT_MSSSP = 13,
+};
+
+// Scan on an explicit stream with caller-provided partials
+// (scan_partial_bytes(n) bytes).
+size_t scan_partial_bytes(int64_t n);
+int exclusive_scan_i32_on(const int32_t* in, int32_t* out, int64_t n, int32_t* total_host, hipStream_t st,
+                          void* partial_ws, size_t ws_bytes);
+
+// SpMM tiles are built on first use and kept with the matrix orientation.
+int ensure_spmm_tiles(DevCSR& m, hipStream_t st);
+size_t spmm_carry_bytes(const DevCSR& m, int type, int k);
+int spmm_run(DevCSR& m, int type, const void* X, void* Y, int k, int add, int mul, int accum, void* carry,
+             hipStream_t st);
+
+size_t merge_carry_bytes(const DevCSR& m);
+int spmv_merge_ws(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum, void* carry);
+
+size_t msbfs_ws_bytes(int64_t n);
+int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t ws_bytes, int32_t* levels_out,
+              int64_t* stats, hipStream_t st);
+
+}  // namespace gbk

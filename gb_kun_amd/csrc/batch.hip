@@ -1,0 +1,581 @@
+// Batched (multi-vector) kernels: many right-hand sides in one launch.
+//
+//   SpMM   Y[n][k] = accum(Y, A (+.*) X[n][k]) for k <= 64 right-hand sides
+//          stored row-major (the k values of one vertex are contiguous, so
+//          one gather of a neighbour fetches all k of them in one coalesced
+//          k*sizeof(T) access).  It is the multi-vector form of the pull SpMV
+//          (src/backend/spmv/spmv_csr.h:341-384): one pass over col/val
+//          serves k vectors.  Work is split by merge path (rows + nonzeros)
+//          into tiles of SPMM_TILE items; one group of LW lanes (LW = k
+//          rounded up to a power of two, at least 8) walks one tile, lane q
+//          owning column q.  A group loads 8 col/val pairs with 8 lanes,
+//          broadcasts them by shuffle and issues the 8 gathers of X before it
+//          folds them, so 8 neighbour rows are in flight per group.  Rows
+//          that cross a tile boundary leave their partial sums in a carry
+//          buffer and a fix-up kernel adds them in tile order, so results do
+//          not depend on scheduling.
+//
+//   MS-BFS 64 breadth-first searches at once (multi-source BFS) as a boolean
+//          SpMM over 64-bit frontier words: bit q of F[v] says v is in the
+//          frontier of search q.  Pull steps fold F over the in-edges of each
+//          vertex that is still unvisited in some search (lor.land over 64
+//          searches in one 64-bit OR) and stop scanning a vertex once every
+//          missing bit has been found; push steps expand the vertices with a
+//          non-empty frontier word, edge-balanced, with 64-bit atomicOr.  The
+//          step direction follows the frontier's edge count as in the single
+//          BFS.  Levels are 1-based with 0 = unreached, exactly as the
+//          single-source driver (algorithms/bfs/bfs.hpp:94-142) reports them.
+#include <algorithm>
+#include <vector>
+
+#include "gbk_objects.h"
+#include "batch.h"
+
+namespace gbk {
+
+
+static dim3 grid_of(int64_t n, int per = 256) {
+    int64_t b = (n + per - 1) / per;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+// ------------------------------------------------------------------ tiles --
+__global__ void k_merge_tiles_sized(const int32_t* __restrict__ rp, int64_t nrows, int64_t nnz, int64_t ntiles,
+                                    int64_t tile, int2* __restrict__ tiles) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    int64_t diag = t * tile;
+    if (diag > nrows + nnz) diag = nrows + nnz;
+    int64_t lo = diag - nnz > 0 ? diag - nnz : 0;
+    int64_t hi = diag < nrows ? diag : nrows;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)rp[mid + 1] <= diag - mid - 1) lo = mid + 1;
+        else hi = mid;
+    }
+    tiles[t] = make_int2((int)lo, (int)(diag - lo));
+}
+
+int ensure_spmm_tiles(DevCSR& m, hipStream_t st) {
+    if (m.spmm_tiles) return GBK_SUCCESS;
+    const int64_t nt = (m.nrows + m.nnz + SPMM_TILE - 1) / SPMM_TILE;
+    GBK_TRY(dev_alloc((void**)&m.spmm_tiles, sizeof(int2) * (nt + 1)));
+    hipLaunchKernelGGL(k_merge_tiles_sized, grid_of(nt + 1), dim3(256), 0, st, m.row_ptr, m.nrows, m.nnz, nt,
+                       (int64_t)SPMM_TILE, m.spmm_tiles);
+    GBK_LAUNCH_CHECK();
+    m.spmm_ntiles = nt;
+    return GBK_SUCCESS;
+}
+
+// ------------------------------------------------------------------ SpMM --
+template <typename T> __device__ __forceinline__ T shfl_t(T v, int src, int width) {
+    if constexpr (sizeof(T) >= 4) return __shfl(v, src, width);
+    else return (T)__shfl((int)v, src, width);
+}
+
+template <typename T, int ADD, int MUL, int LW>
+__global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                              const T* __restrict__ val, const T* __restrict__ X, T* __restrict__ Y,
+                                              int32_t nrows, int K, const int2* __restrict__ tiles, int64_t ntiles,
+                                              T* __restrict__ carry_head, T* __restrict__ carry_tail, int accum) {
+    constexpr int GPB = 256 / LW;
+    const int64_t t = (int64_t)blockIdx.x * GPB + threadIdx.x / LW;
+    if (t >= ntiles) return;  // whole group leaves together
+    const int lane = threadIdx.x % LW;
+    const bool act = lane < K;
+    const int2 c0 = tiles[t], c1 = tiles[t + 1];
+    const T ident = add_identity<ADD, T>();
+    int row = c0.x;
+    const int row_lim = c1.x;
+    int kk = c0.y;
+    const int k_end = c1.y;
+    const bool head_open = (row < row_lim) && (rp[row] < kk);
+    int row_end = (row < nrows) ? rp[row + 1] : k_end;
+    T acc = ident;
+
+    auto flush = [&](int r, T v) {
+        if (!act) return;
+        if (r == c0.x && head_open) {
+            carry_head[t * K + lane] = v;
+        } else {
+            T* py = Y + (int64_t)r * K + lane;
+            *py = (accum == GBK_ACC_NONE) ? v : binop<T>(accum, *py, v);
+        }
+    };
+
+    while (kk < k_end) {
+        const int nb = min(8, k_end - kk);
+        int cc = 0;
+        T vv = T(0);
+        if (lane < nb) {
+            cc = __builtin_nontemporal_load(col + kk + lane);
+            vv = __builtin_nontemporal_load(val + kk + lane);
+        }
+        T xv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int ci = __shfl(cc, i, LW);
+            xv[i] = (i < nb && act) ? X[(int64_t)ci * K + lane] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const T a = shfl_t(vv, i, LW);
+            if (i < nb) {
+                while (row_end <= kk + i) {
+                    flush(row, acc);
+                    acc = ident;
+                    ++row;
+                    row_end = rp[row + 1];
+                }
+                acc = add_op<ADD, T>(acc, mul_op<MUL, T>(a, xv[i]));
+            }
+        }
+        kk += nb;
+    }
+    while (row < row_lim) {
+        flush(row, acc);
+        acc = ident;
+        ++row;
+    }
+    if (act) carry_tail[t * K + lane] = acc;
+}
+
+// Adds the carried partials of every row that begins before its ending tile.
+template <typename T, int ADD, int LW>
+__global__ __launch_bounds__(256) void k_spmm_fixup(const int32_t* __restrict__ rp, const int2* __restrict__ tiles,
+                                                    int64_t ntiles, int K, const T* __restrict__ carry_head,
+                                                    const T* __restrict__ carry_tail, T* __restrict__ Y, int accum) {
+    constexpr int GPB = 256 / LW;
+    const int64_t t = (int64_t)blockIdx.x * GPB + threadIdx.x / LW;
+    const int lane = threadIdx.x % LW;
+    if (t >= ntiles || lane >= K) return;
+    const int2 c0 = tiles[t], c1 = tiles[t + 1];
+    const int r0 = c0.x;
+    if (c1.x == r0) return;
+    const int start = rp[r0];
+    if (start >= c0.y) return;
+    T total = carry_head[t * K + lane];
+    for (int64_t u = t - 1; u >= 0; --u) {
+        total = add_op<ADD, T>(carry_tail[u * K + lane], total);
+        if (tiles[u].y <= start) break;
+    }
+    T* py = Y + (int64_t)r0 * K + lane;
+    *py = (accum == GBK_ACC_NONE) ? total : binop<T>(accum, *py, total);
+}
+
+static bool order_sensitive_add(int add) {
+    return add == GBK_ADD_LESS || add == GBK_ADD_FIRST || add == GBK_ADD_SECOND;
+}
+
+size_t spmm_carry_bytes(const DevCSR& m, int type, int k) {
+    const int64_t nt = (m.nrows + m.nnz + SPMM_TILE - 1) / SPMM_TILE;
+    return 2 * type_size(type) * (size_t)(nt > 0 ? nt : 1) * (size_t)k;
+}
+
+int spmm_run(DevCSR& m, int type, const void* X, void* Y, int k, int add, int mul, int accum, void* carry,
+             hipStream_t st) {
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "SpMM needs 1 <= k <= 64 right-hand sides (got %d)", k);
+    if (order_sensitive_add(add))
+        return fail(GBK_INVALID_VALUE, "SpMM needs a commutative monoid (got add=%d)", add);
+    GBK_TRY(ensure_spmm_tiles(m, st));
+    const int64_t nt = m.spmm_ntiles;
+    if (nt == 0) return GBK_SUCCESS;
+    return dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        T* head = (T*)carry;
+        T* tail = head + nt * k;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
+            if constexpr (ADD == GBK_ADD_LESS || ADD == GBK_ADD_FIRST || ADD == GBK_ADD_SECOND) {
+                return GBK_INVALID_VALUE;
+            } else {
+#define GBK_SPMM(LWV)                                                                                           \
+    {                                                                                                           \
+        constexpr int GPB = 256 / LWV;                                                                          \
+        hipLaunchKernelGGL((k_spmm<T, ADD, MUL, LWV>), grid_of(nt, GPB), dim3(256), 0, st, m.row_ptr, m.col,     \
+                           (const T*)m.val, (const T*)X, (T*)Y, (int32_t)m.nrows, k, m.spmm_tiles, nt, head,     \
+                           tail, accum);                                                                        \
+        GBK_LAUNCH_CHECK();                                                                                     \
+        hipLaunchKernelGGL((k_spmm_fixup<T, ADD, LWV>), grid_of(nt, GPB), dim3(256), 0, st, m.row_ptr,           \
+                           m.spmm_tiles, nt, k, (const T*)head, (const T*)tail, (T*)Y, accum);                  \
+        GBK_LAUNCH_CHECK();                                                                                     \
+        return GBK_SUCCESS;                                                                                     \
+    }
+                if (k <= 8) GBK_SPMM(8)
+                if (k <= 16) GBK_SPMM(16)
+                if (k <= 32) GBK_SPMM(32)
+                GBK_SPMM(64)
+#undef GBK_SPMM
+            }
+        });
+    });
+}
+
+// ---------------------------------------------------------------- MS-BFS --
+__device__ __forceinline__ unsigned long long lanemask_lt_b() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__global__ void k_msbfs_init(uint64_t* __restrict__ F, uint64_t* __restrict__ V, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        F[i] = 0;
+        V[i] = 0;
+    }
+}
+
+// One thread per search: sets the source bit; levels[v*64 + q] = 1.
+__global__ void k_msbfs_sources(const int32_t* __restrict__ src, int nsrc, uint64_t* __restrict__ F,
+                                uint64_t* __restrict__ V, int32_t* __restrict__ lev) {
+    const int q = threadIdx.x;
+    if (q < nsrc) {
+        const int s = src[q];
+        atomicOr((unsigned long long*)(F + s), 1ull << q);
+        atomicOr((unsigned long long*)(V + s), 1ull << q);
+        lev[(int64_t)s * 64 + q] = 1;
+    }
+}
+
+// Pull step: G lanes per vertex.  A vertex still missing some search bits
+// ORs the frontier words of its in-neighbours, stopping once every missing
+// bit is found.  Writes the new frontier word of every vertex (0 if none).
+template <int G>
+__global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ crp, const int32_t* __restrict__ ccol,
+                                                    int64_t n, uint64_t all, const uint64_t* __restrict__ F,
+                                                    uint64_t* __restrict__ V, uint64_t* __restrict__ N,
+                                                    int32_t* __restrict__ lev, int32_t next_level,
+                                                    unsigned long long* __restrict__ counters) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    uint64_t found = 0;
+    uint64_t vis = all;
+    if (v < n) {
+        vis = V[v];
+        const uint64_t want = all & ~vis;
+        if (want) {
+            const int e1 = crp[v + 1];
+            int e = crp[v] + lane;
+            for (; e < e1; e += G) {
+                found |= F[ccol[e]] & want;
+                if (found == want) break;
+            }
+        }
+    }
+    // OR across the G lanes of the group
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) found |= (uint64_t)__shfl_xor((unsigned long long)found, off, G);
+    }
+    if (v < n) {
+        if (lane == 0) {
+            N[v] = found;
+            if (found) V[v] = vis | found;
+        }
+        // level writes: lanes of the group split the 64 search bits
+        if (found) {
+            for (int q = lane; q < 64; q += G)
+                if ((found >> q) & 1ull) lev[v * 64 + q] = next_level;
+        }
+    }
+    unsigned long long c = (v < n && lane == 0 && found) ? (unsigned long long)__popcll(found) : 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(counters, c);
+}
+
+// Compacts the vertices with a non-empty frontier word into a queue and sums
+// their out-degrees (the push/pull decision and the push expansion use it).
+__global__ __launch_bounds__(256) void k_msbfs_queue(const int32_t* __restrict__ rp, const uint64_t* __restrict__ F,
+                                                     int64_t n, int32_t* __restrict__ q, int32_t* __restrict__ qn,
+                                                     unsigned long long* __restrict__ edges) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = v < n && F[v] != 0;
+    unsigned long long d = take ? (unsigned long long)(rp[v + 1] - rp[v]) : 0ull;
+    const unsigned long long b = __ballot(take);
+    const int lane = threadIdx.x & 63;
+    if (b) {
+        const int leader = __ffsll((long long)b) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(qn, (int)__popcll(b));
+        base = __shfl(base, leader, 64);
+        if (take) q[base + __popcll(b & lanemask_lt_b())] = (int32_t)v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+    if (lane == 0 && d) atomicAdd(edges, d);
+}
+
+__global__ void k_queue_degrees(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
+                                int32_t* __restrict__ deg) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        const int u = q[i];
+        deg[i] = rp[u + 1] - rp[u];
+    }
+}
+
+// Push step: thread e owns edge e of the queue's concatenated adjacency.
+__global__ __launch_bounds__(256) void k_msbfs_push(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ q, int64_t nq,
+                                                    const int32_t* __restrict__ offs, int64_t nedges,
+                                                    const uint64_t* __restrict__ F, const uint64_t* __restrict__ V,
+                                                    uint64_t* __restrict__ N) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nedges) return;
+    int64_t lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const int u = q[lo];
+    const int v = col[rp[u] + (e - offs[lo])];
+    const uint64_t bits = F[u] & ~V[v];
+    if (bits) atomicOr((unsigned long long*)(N + v), (unsigned long long)bits);
+}
+
+// After a push step: new = N & ~V per vertex; marks visited and levels.
+__global__ __launch_bounds__(256) void k_msbfs_settle(int64_t n, uint64_t* __restrict__ V, uint64_t* __restrict__ N,
+                                                      int32_t* __restrict__ lev, int32_t next_level,
+                                                      unsigned long long* __restrict__ counters) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long c = 0;
+    if (v < n) {
+        const uint64_t vis = V[v];
+        const uint64_t nw = N[v] & ~vis;
+        N[v] = nw;
+        if (nw) {
+            V[v] = vis | nw;
+            uint64_t b = nw;
+            while (b) {
+                const int qb = __ffsll((long long)b) - 1;
+                lev[v * 64 + qb] = next_level;
+                b &= b - 1;
+            }
+            c = (unsigned long long)__popcll(nw);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(counters, c);
+}
+
+__global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// lev[v*64 + q] -> out[q*n + v] for q < nsrc.
+__global__ void k_msbfs_transpose(const int32_t* __restrict__ lev, int64_t n, int nsrc, int32_t* __restrict__ out) {
+    __shared__ int32_t tile[64][65];
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 256 threads: 4 rows per pass
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t v = v0 + r;
+        tile[r][tx] = (v < n) ? lev[v * 64 + tx] : 0;
+    }
+    __syncthreads();
+    for (int q = ty; q < nsrc; q += 4) {
+        const int64_t v = v0 + tx;
+        if (v < n) out[(int64_t)q * n + v] = tile[tx][q];
+    }
+}
+
+// Workspace of one multi-source BFS over n vertices, carved in this order
+// (each piece 256-byte aligned): F, V, N (u64[n]); levels int32[n*64];
+// queue, degrees, offsets int32[n+1]; scan partials; 3 counters.
+struct MsbfsLayout {
+    uint64_t *F, *V, *N;
+    int32_t *lev, *q, *deg, *offs;
+    void* partial;
+    size_t part_bytes;
+    unsigned long long* ctr;
+    size_t total;
+};
+
+static size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static MsbfsLayout msbfs_layout(int64_t n, char* base) {
+    MsbfsLayout L;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* r = base ? base + off : nullptr;
+        off = up256(off + bytes);
+        return r;
+    };
+    L.F = (uint64_t*)take(sizeof(uint64_t) * n);
+    L.V = (uint64_t*)take(sizeof(uint64_t) * n);
+    L.N = (uint64_t*)take(sizeof(uint64_t) * n);
+    L.lev = (int32_t*)take(sizeof(int32_t) * 64 * (size_t)n);
+    L.q = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.deg = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.offs = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.part_bytes = scan_partial_bytes(n);
+    L.partial = take(L.part_bytes);
+    L.ctr = (unsigned long long*)take(4 * sizeof(unsigned long long));
+    L.total = off;
+    return L;
+}
+
+size_t msbfs_ws_bytes(int64_t n) { return msbfs_layout(n, nullptr).total + 256; }
+
+int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t ws_bytes, int32_t* levels_out,
+              int64_t* stats, hipStream_t st) {
+    const int64_t n = A->n;
+    if (nsrc < 1 || nsrc > 64) return fail(GBK_INVALID_VALUE, "multi-source BFS takes 1..64 sources (got %d)", nsrc);
+    if (ws_bytes < msbfs_ws_bytes(n)) return fail(GBK_INSUFFICIENT_SPACE, "multi-source BFS workspace too small");
+    const DevCSR& out = A->csr;
+    const DevCSR& in = A->csc;
+    char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    MsbfsLayout L = msbfs_layout(n, base);
+    uint64_t* F = L.F;
+    uint64_t* V = L.V;
+    uint64_t* N = L.N;
+    int32_t* lev = L.lev;
+    int32_t* q = L.q;
+    int32_t* deg = L.deg;
+    int32_t* offs = L.offs;
+    void* partial = L.partial;
+    const size_t part_bytes = L.part_bytes;
+    unsigned long long* ctr = L.ctr;  // [0] new bits, [1] frontier edges, [2] queue length
+    const uint64_t all = nsrc == 64 ? ~0ull : ((1ull << nsrc) - 1);
+
+    hipLaunchKernelGGL(k_msbfs_init, grid_of(n), dim3(256), 0, st, F, V, n);
+    GBK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zero_u64, grid_of(32 * n), dim3(256), 0, st, (uint64_t*)lev, 32 * n);
+    GBK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_msbfs_sources, dim3(1), dim3(64), 0, st, src_dev, nsrc, F, V, lev);
+    GBK_LAUNCH_CHECK();
+
+    int G = 4;
+    {
+        const double avg = n ? (double)in.nnz / (double)n : 0;
+        while (G < 32 && G < avg) G <<= 1;
+    }
+    int64_t levels = 1, pushes = 0, pulls = 0, reached = nsrc, edges_scanned = 0;
+    int32_t cur = 1;
+    unsigned long long h[3];
+    while (true) {
+        // frontier queue and its out-edge count
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (uint64_t*)ctr, (int64_t)3);
+        GBK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_msbfs_queue, grid_of(n), dim3(256), 0, st, out.row_ptr, F, n, q, (int32_t*)(ctr + 2),
+                           ctr + 1);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        const int64_t nq = (int64_t)(h[2] & 0xffffffffull);
+        const unsigned long long m_f = h[1];
+        if (nq == 0) break;
+        edges_scanned += (int64_t)m_f;
+        const bool push = (double)m_f * 14.0 < (double)out.nnz;
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (uint64_t*)ctr, (int64_t)1);
+        GBK_LAUNCH_CHECK();
+        if (push) {
+            ++pushes;
+            hipLaunchKernelGGL(k_zero_u64, grid_of(n), dim3(256), 0, st, N, n);
+            GBK_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_queue_degrees, grid_of(nq), dim3(256), 0, st, out.row_ptr, q, nq, deg);
+            GBK_LAUNCH_CHECK();
+            int32_t ne = 0;
+            GBK_TRY(exclusive_scan_i32_on(deg, offs, nq, &ne, st, partial, part_bytes));
+            if (ne > 0) {
+                hipLaunchKernelGGL(k_msbfs_push, grid_of(ne), dim3(256), 0, st, out.row_ptr, out.col, q, nq, offs,
+                                   (int64_t)ne, F, V, N);
+                GBK_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(k_msbfs_settle, grid_of(n), dim3(256), 0, st, n, V, N, lev, cur + 1, ctr);
+            GBK_LAUNCH_CHECK();
+        } else {
+            ++pulls;
+#define GBK_MSPULL(GG)                                                                                         \
+    case GG:                                                                                                   \
+        hipLaunchKernelGGL(k_msbfs_pull<GG>, grid_of(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, all, F, V, \
+                           N, lev, cur + 1, ctr);                                                              \
+        break;
+            switch (G) {
+                GBK_MSPULL(4)
+                GBK_MSPULL(8)
+                GBK_MSPULL(16)
+                GBK_MSPULL(32)
+            }
+#undef GBK_MSPULL
+            GBK_LAUNCH_CHECK();
+        }
+        unsigned long long newbits = 0;
+        GBK_HIP(hipMemcpyAsync(&newbits, ctr, sizeof(newbits), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        if (newbits == 0) break;
+        reached += (int64_t)newbits;
+        std::swap(F, N);
+        ++cur;
+        ++levels;
+    }
+    hipLaunchKernelGGL(k_msbfs_transpose, grid_of(n, 64), dim3(256), 0, st, lev, n, nsrc, levels_out);
+    GBK_LAUNCH_CHECK();
+    if (stats) {
+        stats[0] = levels;
+        stats[1] = pushes;
+        stats[2] = pulls;
+        stats[3] = edges_scanned;
+        stats[4] = reached;
+    }
+    return GBK_SUCCESS;
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+extern "C" int gbk_spmm_device(gbk_matrix A, int transposed, const void* X, void* Y, int k, gbk_semiring op,
+                               int accum) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!X || !Y) return fail(GBK_NULL_POINTER, "null dense block pointer");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (X == Y) return fail(GBK_INVALID_VALUE, "SpMM input and output blocks must not alias");
+    if (accum < GBK_ACC_NONE || accum > GBK_ACC_NE) return fail(GBK_INVALID_VALUE, "unknown accumulator %d", accum);
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "SpMM needs 1 <= k <= 64 right-hand sides (got %d)", k);
+    GBK_TRY(ensure_init());
+    DevCSR& m = transposed ? A->csc : A->csr;
+    void* carry;
+    GBK_TRY(scratch(SLOT_SPMM_CARRY, spmm_carry_bytes(m, A->type, k), &carry));
+    return spmm_run(m, A->type, X, Y, k, op.add, op.mul, accum, carry, stream());
+}
+
+extern "C" int gbk_msbfs_device(gbk_matrix A, const int64_t* sources, int nsrc, int32_t* levels_dev, int64_t* stats) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!sources || !levels_dev) return fail(GBK_NULL_POINTER, "null pointer");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (nsrc < 1 || nsrc > 64) return fail(GBK_INVALID_VALUE, "multi-source BFS takes 1..64 sources (got %d)", nsrc);
+    std::vector<int32_t> s32(nsrc);
+    for (int i = 0; i < nsrc; ++i) {
+        if (sources[i] < 0 || sources[i] >= A->n)
+            return fail(GBK_INVALID_INDEX, "source %lld out of range [0, %lld)", (long long)sources[i],
+                        (long long)A->n);
+        s32[i] = (int32_t)sources[i];
+    }
+    GBK_TRY(ensure_init());
+    hipStream_t st = stream();
+    void *ws, *src_dev;
+    GBK_TRY(scratch(SLOT_MSBFS, msbfs_ws_bytes(A->n), &ws));
+    GBK_TRY(scratch(SLOT_SOURCES, sizeof(int32_t) * 64, &src_dev));
+    GBK_HIP(hipMemcpyAsync(src_dev, s32.data(), sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
+    return msbfs_run(A, (const int32_t*)src_dev, nsrc, ws, msbfs_ws_bytes(A->n), levels_dev, stats, st);
+}
+
+extern "C" int gbk_msbfs(gbk_matrix A, const int64_t* sources, int nsrc, int32_t* levels, int64_t* stats) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!levels) return fail(GBK_NULL_POINTER, "null levels");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (nsrc < 1 || nsrc > 64) return fail(GBK_INVALID_VALUE, "multi-source BFS takes 1..64 sources (got %d)", nsrc);
+    GBK_TRY(ensure_init());
+    void* out;
+    GBK_TRY(scratch(SLOT_MSBFS_OUT, sizeof(int32_t) * (size_t)nsrc * (size_t)A->n, &out));
+    GBK_TRY(gbk_msbfs_device(A, sources, nsrc, (int32_t*)out, stats));
+    GBK_HIP(hipMemcpyAsync(levels, out, sizeof(int32_t) * (size_t)nsrc * (size_t)A->n, hipMemcpyDeviceToHost,
+                           stream()));
+    GBK_HIP(hipStreamSynchronize(stream()));
+    return GBK_SUCCESS;
+}

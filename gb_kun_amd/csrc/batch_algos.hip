@@ -1,0 +1,520 @@
+// Multi-query graph drivers on the batched kernels: personalized PageRank
+// for k sources at once (one SpMM per iteration instead of k SpMVs) and
+// min.plus SSSP for k sources at once.
+//
+// Personalized PageRank (the per-source form of LAGraph_page_rank_sinks,
+// algorithms/pr/pr.hpp:20-153): on a matrix whose entry (v, u) is the
+// transition probability 1/outdeg(u) of edge u -> v,
+//     r_q' = d * A r_q + ((1 - d) + d * sink_q) e_{s_q},
+//     sink_q = sum of r_q over vertices without out-edges,
+// iterated per column until the L1 change of that column drops below tol
+// (columns that converge are frozen while the others continue).  With a
+// uniform start and teleport vector instead of e_{s_q} this is the
+// reference's PageRank with sinks.
+//
+// Multi-source SSSP (sssp_bellman_ford_blast, algorithms/sssp/sssp_blast.hpp:8-72,
+// for k sources): the Bellman-Ford fixed point computed over the set of
+// vertices improved in the last round; a lane group of k lanes relaxes one
+// edge for all k sources, int32 distances saturate at INT32_MAX.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "batch.h"
+#include "gbk_objects.h"
+
+namespace gbk {
+
+static dim3 grid_b(int64_t n, int per = 256) {
+    int64_t b = (n + per - 1) / per;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+static size_t up256b(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------------------- PPR --
+__global__ void k_ppr_init(float* __restrict__ R, int64_t n, int k, const int32_t* __restrict__ src) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * k) {
+        const int q = (int)(i % k);
+        const int64_t v = i / k;
+        R[i] = (src[q] == v) ? 1.0f : 0.0f;
+    }
+}
+
+// out-degree = entries in the column = row length of the CSC
+__global__ void k_sink_flags(const int32_t* __restrict__ crp, int64_t n, int32_t* __restrict__ flag) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) flag[v] = (crp[v + 1] == crp[v]) ? 1 : 0;
+}
+
+__global__ void k_compact_flags(const int32_t* __restrict__ flag, const int32_t* __restrict__ offs, int64_t n,
+                                int32_t* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && flag[v]) out[offs[v]] = (int32_t)v;
+}
+
+// sink[q] = sum over sinks of R[v][q] (fp64 accumulation, one atomic per
+// wave-column pair).  Block = 256 threads = 256/k sinks x k columns.
+__global__ __launch_bounds__(256) void k_ppr_sink(const float* __restrict__ R, int k, const int32_t* __restrict__ sinks,
+                                                  int64_t nsinks, double* __restrict__ sink) {
+    __shared__ double s[256];
+    const int per = 256 / k;
+    const int q = threadIdx.x % k;
+    const int j = threadIdx.x / k;
+    double acc = 0.0;
+    if (j < per) {
+        for (int64_t i = (int64_t)blockIdx.x * per + j; i < nsinks; i += (int64_t)gridDim.x * per)
+            acc += (double)R[(int64_t)sinks[i] * k + q];
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double t = 0.0;
+        for (int jj = 0; jj < per; ++jj) t += s[jj * k + threadIdx.x];
+        atomicAdd(sink + threadIdx.x, t);
+    }
+}
+
+// Rn = d * Y + teleport on the source rows; frozen columns keep R.  Adds the
+// L1 change of every column to delta[q].
+// Rn may be Y (each element is read, then written, by the same thread).
+__global__ __launch_bounds__(256) void k_ppr_update(const float* __restrict__ R, const float* Y, float* Rn, int64_t n,
+                                                    int k,
+                                                    const int32_t* __restrict__ src, const double* __restrict__ sink,
+                                                    const int32_t* __restrict__ done, float damping,
+                                                    double* __restrict__ delta) {
+    __shared__ double s[256];
+    const int per = 256 / k;
+    const int q = threadIdx.x % k;
+    const int j = threadIdx.x / k;
+    double acc = 0.0;
+    if (j < per) {
+        const float tele = (float)((1.0 - (double)damping) + (double)damping * sink[q]);
+        const int sq = src[q];
+        const bool frozen = done[q] != 0;
+        for (int64_t v = (int64_t)blockIdx.x * per + j; v < n; v += (int64_t)gridDim.x * per) {
+            const int64_t i = v * k + q;
+            const float old = R[i];
+            float nv;
+            if (frozen) {
+                nv = old;
+            } else {
+                nv = damping * Y[i];
+                if (v == sq) nv += tele;
+            }
+            Rn[i] = nv;
+            acc += fabs((double)nv - (double)old);
+        }
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double t = 0.0;
+        for (int jj = 0; jj < per; ++jj) t += s[jj * k + threadIdx.x];
+        atomicAdd(delta + threadIdx.x, t);
+    }
+}
+
+__global__ void k_zero_f64(double* __restrict__ p, int n) {
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// R[v][q] -> out[q][v]
+template <typename T>
+__global__ void k_transpose_nk(const T* __restrict__ R, int64_t n, int k, T* __restrict__ out) {
+    __shared__ T tile[64][65];
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t v = v0 + r;
+        tile[r][tx] = (v < n && tx < k) ? R[v * k + tx] : T(0);
+    }
+    __syncthreads();
+    for (int q = ty; q < k; q += 4) {
+        const int64_t v = v0 + tx;
+        if (v < n) out[(int64_t)q * n + v] = tile[tx][q];
+    }
+}
+
+struct PprLayout {
+    float *R, *Y;
+    void* carry;
+    int32_t *flag, *offs, *sinks;
+    void* partial;
+    size_t part_bytes;
+    double *sink, *delta;
+    int32_t* done;
+    int32_t* src;
+    size_t total;
+};
+
+static PprLayout ppr_layout(const DevCSR& m, int64_t n, int k, char* base) {
+    PprLayout L;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* r = base ? base + off : nullptr;
+        off = up256b(off + bytes);
+        return r;
+    };
+    L.R = (float*)take(sizeof(float) * (size_t)n * k);
+    L.Y = (float*)take(sizeof(float) * (size_t)n * k);
+    L.carry = take(k == 1 ? merge_carry_bytes(m) : spmm_carry_bytes(m, GBK_FP32, k));
+    L.flag = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.offs = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.sinks = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.part_bytes = scan_partial_bytes(n);
+    L.partial = take(L.part_bytes);
+    L.sink = (double*)take(sizeof(double) * 64);
+    L.delta = (double*)take(sizeof(double) * 64);
+    L.done = (int32_t*)take(sizeof(int32_t) * 64);
+    L.src = (int32_t*)take(sizeof(int32_t) * 64);
+    L.total = off;
+    return L;
+}
+
+size_t ppr_ws_bytes(gbk_matrix A, int k) { return ppr_layout(A->csr, A->n, k, nullptr).total + 256; }
+
+// ranks_out: k*n floats, source-major, on the device.  iters[q]: iterations
+// column q ran (itermax if it did not converge).
+int ppr_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_bytes, float* ranks_out, int itermax,
+            double damping, double tol, int* iters, hipStream_t st) {
+    const int64_t n = A->n;
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "personalized PageRank takes 1..64 sources (got %d)", k);
+    if (A->type != GBK_FP32) return fail(GBK_DOMAIN_MISMATCH, "personalized PageRank needs an FP32 matrix");
+    if (ws_bytes < ppr_ws_bytes(A, k)) return fail(GBK_INSUFFICIENT_SPACE, "PageRank workspace too small");
+    DevCSR& m = A->csr;
+    char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    PprLayout L = ppr_layout(m, n, k, base);
+    GBK_HIP(hipMemcpyAsync(L.src, src_host, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ppr_init, grid_b(n * k), dim3(256), 0, st, L.R, n, k, L.src);
+    GBK_LAUNCH_CHECK();
+    // sink list
+    hipLaunchKernelGGL(k_sink_flags, grid_b(n), dim3(256), 0, st, A->csc.row_ptr, n, L.flag);
+    GBK_LAUNCH_CHECK();
+    int32_t nsinks = 0;
+    GBK_TRY(exclusive_scan_i32_on(L.flag, L.offs, n, &nsinks, st, L.partial, L.part_bytes));
+    hipLaunchKernelGGL(k_compact_flags, grid_b(n), dim3(256), 0, st, L.flag, L.offs, n, L.sinks);
+    GBK_LAUNCH_CHECK();
+    std::vector<int32_t> done(k, 0);
+    for (int q = 0; q < k; ++q) iters[q] = itermax;
+    GBK_HIP(hipMemcpyAsync(L.done, done.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+    const int per = 256 / k;
+    const int64_t ub = std::min<int64_t>((n + per - 1) / per, (int64_t)device_cu_count() * 32);
+    const int64_t sb = std::min<int64_t>(((int64_t)nsinks + per - 1) / per, (int64_t)device_cu_count() * 8);
+    std::vector<double> delta(k);
+    float* R = L.R;
+    float* Y = L.Y;
+    int remaining = k;
+    hipStream_t prev = stream();
+    set_thread_stream(st);
+    int rc = GBK_SUCCESS;
+    for (int it = 1; it <= itermax && remaining > 0; ++it) {
+        if (k == 1) rc = spmv_merge_ws(m, GBK_FP32, R, Y, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L.carry);
+        else rc = spmm_run(m, GBK_FP32, R, Y, k, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L.carry, st);
+        if (rc != GBK_SUCCESS) break;
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, L.sink, 64);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, L.delta, 64);
+        if (nsinks > 0 && sb > 0)
+            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st, R, k, L.sinks, (int64_t)nsinks,
+                               L.sink);
+        // Y is overwritten with the new ranks, then R and Y swap roles
+        hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)std::max<int64_t>(ub, 1)), dim3(256), 0, st, R, Y, Y, n, k,
+                           L.src, L.sink, L.done, (float)damping, L.delta);
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(GBK_PANIC, "PageRank kernel launch failed");
+            break;
+        }
+        if (hipMemcpyAsync(delta.data(), L.delta, sizeof(double) * k, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            rc = fail(GBK_PANIC, "PageRank delta copy failed");
+            break;
+        }
+        std::swap(R, Y);
+        bool changed = false;
+        for (int q = 0; q < k; ++q) {
+            if (!done[q] && tol > 0.0 && delta[q] < tol) {
+                done[q] = 1;
+                iters[q] = it;
+                --remaining;
+                changed = true;
+            }
+        }
+        if (changed && remaining > 0)
+            if (hipMemcpyAsync(L.done, done.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st) != hipSuccess) {
+                rc = fail(GBK_PANIC, "PageRank flag copy failed");
+                break;
+            }
+    }
+    set_thread_stream(prev);
+    GBK_TRY(rc);
+    if (k == 1) {
+        GBK_HIP(hipMemcpyAsync(ranks_out, R, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    } else {
+        hipLaunchKernelGGL(k_transpose_nk<float>, grid_b(n, 64), dim3(256), 0, st, R, n, k, ranks_out);
+        GBK_LAUNCH_CHECK();
+    }
+    return GBK_SUCCESS;
+}
+
+// ------------------------------------------------------------ multi-SSSP --
+template <int LW>
+__global__ __launch_bounds__(256) void k_mssp_relax(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ w, const int32_t* __restrict__ q,
+                                                    int64_t nq, const int32_t* __restrict__ offs, int64_t nedges,
+                                                    int K, int32_t* __restrict__ D, int32_t* __restrict__ inq,
+                                                    int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count,
+                                                    int32_t round, unsigned long long* __restrict__ changed) {
+    constexpr int GPB = 256 / LW;
+    constexpr int EPG = 4;  // edges per group
+    const int64_t g = (int64_t)blockIdx.x * GPB + threadIdx.x / LW;
+    const int lane = threadIdx.x % LW;
+    const int64_t e0 = g * EPG;
+    bool take_any = false;
+    int32_t vtake = 0;
+    unsigned long long chg = 0;
+    if (e0 < nedges) {
+        int64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= e0) lo = mid;
+            else hi = mid - 1;
+        }
+        int64_t i = lo;
+        const int64_t e1 = min(e0 + EPG, nedges);
+        for (int64_t e = e0; e < e1; ++e) {
+            while (i + 1 < nq && offs[i + 1] <= e) ++i;
+            const int u = q[i];
+            const int64_t kk = rp[u] + (e - offs[i]);
+            const int v = col[kk];
+            const int32_t wt = w[kk];
+            bool imp = false;
+            if (lane < K) {
+                const int32_t du = D[(int64_t)u * K + lane];
+                if (du != INT32_MAX) {
+                    const int32_t nd = sat_add<int32_t>(du, wt);
+                    int32_t* pv = D + (int64_t)v * K + lane;
+                    if (nd < *pv && atomicMin(pv, nd) > nd) imp = true;
+                }
+            }
+            const unsigned long long b = __ballot(imp);
+            // bits of this group only
+            const int gbase = (threadIdx.x & 63) & ~(LW - 1);
+            const unsigned long long gm = (LW == 64) ? ~0ull : (((1ull << LW) - 1) << gbase);
+            const unsigned long long mine = b & gm;
+            if (mine) {
+                chg |= (mine >> gbase);
+                if (lane == 0 && atomicExch(inq + v, round) != round) {
+                    take_any = true;
+                    vtake = v;
+                }
+            }
+            // one enqueue slot per thread: flush it before the next edge
+            {
+                const unsigned long long tb = __ballot(take_any);
+                if (tb) {
+                    const int wl = threadIdx.x & 63;
+                    const int leader = __ffsll((long long)tb) - 1;
+                    int basei = 0;
+                    if (wl == leader) basei = atomicAdd(nq_count, (int)__popcll(tb));
+                    basei = __shfl(basei, leader, 64);
+                    if (take_any) {
+                        const unsigned long long lt = wl == 0 ? 0ull : (~0ull >> (64 - wl));
+                        nq_out[basei + __popcll(tb & lt)] = vtake;
+                    }
+                }
+                take_any = false;
+            }
+        }
+    }
+    // columns that improved somewhere this round (bit q = source q)
+    if (lane == 0 && chg) atomicOr(changed, chg);
+}
+
+__global__ void k_mssp_init(int32_t* __restrict__ D, int64_t n, int K, int32_t* __restrict__ inq) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * K) D[i] = INT32_MAX;
+    if (i < n) inq[i] = -1;
+}
+
+__global__ void k_mssp_sources(int32_t* __restrict__ D, int K, const int32_t* __restrict__ src,
+                               int32_t* __restrict__ q, int32_t* __restrict__ qn, int32_t* __restrict__ inq) {
+    if (threadIdx.x != 0) return;
+    int cnt = 0;
+    for (int s = 0; s < K; ++s) {
+        const int v = src[s];
+        D[(int64_t)v * K + s] = 0;
+        if (inq[v] != -2) {
+            inq[v] = -2;
+            q[cnt++] = v;
+        }
+    }
+    for (int s = 0; s < K; ++s) inq[src[s]] = -1;
+    *qn = cnt;
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ p, int n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ void k_queue_deg(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
+                            int32_t* __restrict__ deg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        const int u = q[i];
+        deg[i] = rp[u + 1] - rp[u];
+    }
+}
+
+struct MsspLayout {
+    int32_t *D, *inq, *q0, *q1, *deg, *offs, *cnt, *src;
+    void* partial;
+    size_t part_bytes;
+    unsigned long long* changed;
+    size_t total;
+};
+
+static MsspLayout mssp_layout(int64_t n, int k, char* base) {
+    MsspLayout L;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* r = base ? base + off : nullptr;
+        off = up256b(off + bytes);
+        return r;
+    };
+    L.D = (int32_t*)take(sizeof(int32_t) * (size_t)n * k);
+    L.inq = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.q0 = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.q1 = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.deg = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.offs = (int32_t*)take(sizeof(int32_t) * (n + 1));
+    L.part_bytes = scan_partial_bytes(n);
+    L.partial = take(L.part_bytes);
+    L.cnt = (int32_t*)take(sizeof(int32_t) * 4);
+    L.src = (int32_t*)take(sizeof(int32_t) * 64);
+    L.changed = (unsigned long long*)take(sizeof(unsigned long long) * 2);
+    L.total = off;
+    return L;
+}
+
+size_t mssp_ws_bytes(int64_t n, int k) { return mssp_layout(n, k, nullptr).total + 256; }
+
+// dist_out: k*n int32, source-major, on the device.
+int mssp_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_bytes, int32_t* dist_out,
+             int64_t* stats, hipStream_t st) {
+    const int64_t n = A->n;
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "multi-source SSSP takes 1..64 sources (got %d)", k);
+    if (A->type != GBK_INT32) return fail(GBK_DOMAIN_MISMATCH, "multi-source SSSP needs an INT32 weight matrix");
+    if (ws_bytes < mssp_ws_bytes(n, k)) return fail(GBK_INSUFFICIENT_SPACE, "SSSP workspace too small");
+    const DevCSR& m = A->csr;
+    char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    MsspLayout L = mssp_layout(n, k, base);
+    GBK_HIP(hipMemcpyAsync(L.src, src_host, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_mssp_init, grid_b(std::max<int64_t>(n * k, n)), dim3(256), 0, st, L.D, n, k, L.inq);
+    GBK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L.cnt, 4);
+    GBK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mssp_sources, dim3(1), dim3(64), 0, st, L.D, k, L.src, L.q0, L.cnt, L.inq);
+    GBK_LAUNCH_CHECK();
+    int32_t nq32 = 0;
+    GBK_HIP(hipMemcpyAsync(&nq32, L.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    int64_t nq = nq32, rounds = 0, relaxed = 0;
+    int32_t* qa = L.q0;
+    int32_t* qb = L.q1;
+    const int LW = k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
+    while (nq > 0) {
+        hipLaunchKernelGGL(k_queue_deg, grid_b(nq), dim3(256), 0, st, m.row_ptr, qa, nq, L.deg);
+        GBK_LAUNCH_CHECK();
+        int32_t ne = 0;
+        GBK_TRY(exclusive_scan_i32_on(L.deg, L.offs, nq, &ne, st, L.partial, L.part_bytes));
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L.cnt, 4);
+        GBK_LAUNCH_CHECK();
+        if (ne > 0) {
+            const int64_t groups = ((int64_t)ne + 3) / 4;
+#define GBK_MSSP(LWV)                                                                                          \
+    case LWV:                                                                                                  \
+        hipLaunchKernelGGL(k_mssp_relax<LWV>, grid_b(groups, 256 / LWV), dim3(256), 0, st, m.row_ptr, m.col,    \
+                           (const int32_t*)m.val, qa, nq, L.offs, (int64_t)ne, k, L.D, L.inq, qb, L.cnt,         \
+                           (int32_t)rounds, L.changed);                                                        \
+        break;
+            switch (LW) {
+                GBK_MSSP(8)
+                GBK_MSSP(16)
+                GBK_MSSP(32)
+                GBK_MSSP(64)
+            }
+#undef GBK_MSSP
+            GBK_LAUNCH_CHECK();
+        }
+        relaxed += ne;
+        GBK_HIP(hipMemcpyAsync(&nq32, L.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+        nq = nq32;
+        std::swap(qa, qb);
+        ++rounds;
+    }
+    hipLaunchKernelGGL(k_transpose_nk<int32_t>, grid_b(n, 64), dim3(256), 0, st, L.D, n, k, dist_out);
+    GBK_LAUNCH_CHECK();
+    if (stats) {
+        stats[0] = rounds;
+        stats[1] = relaxed;
+    }
+    return GBK_SUCCESS;
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+static int check_sources(gbk_matrix A, const int64_t* sources, int k, std::vector<int32_t>& s32) {
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "a batch takes 1..64 sources (got %d)", k);
+    if (!sources) return fail(GBK_NULL_POINTER, "null sources");
+    s32.resize(k);
+    for (int i = 0; i < k; ++i) {
+        if (sources[i] < 0 || sources[i] >= A->n)
+            return fail(GBK_INVALID_INDEX, "source %lld out of range [0, %lld)", (long long)sources[i],
+                        (long long)A->n);
+        s32[i] = (int32_t)sources[i];
+    }
+    return GBK_SUCCESS;
+}
+
+extern "C" int gbk_ppr_multi(gbk_matrix A, const int64_t* sources, int k, float* ranks, int itermax, double damping,
+                             double tol, int* iters) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (!ranks || !iters) return fail(GBK_NULL_POINTER, "null output");
+    if (itermax < 1) return fail(GBK_INVALID_VALUE, "itermax must be positive");
+    std::vector<int32_t> s32;
+    GBK_TRY(check_sources(A, sources, k, s32));
+    GBK_TRY(ensure_init());
+    hipStream_t st = stream();
+    void *ws, *out;
+    const size_t wb = ppr_ws_bytes(A, k);
+    GBK_TRY(scratch(SLOT_PPR, wb, &ws));
+    GBK_TRY(scratch(SLOT_MSBFS_OUT, sizeof(float) * (size_t)k * A->n, &out));
+    GBK_TRY(ppr_run(A, s32.data(), k, ws, wb, (float*)out, itermax, damping, tol, iters, st));
+    GBK_HIP(hipMemcpyAsync(ranks, out, sizeof(float) * (size_t)k * A->n, hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}
+
+extern "C" int gbk_sssp_multi(gbk_matrix A, const int64_t* sources, int k, int32_t* dist, int64_t* stats) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (!dist) return fail(GBK_NULL_POINTER, "null output");
+    std::vector<int32_t> s32;
+    GBK_TRY(check_sources(A, sources, k, s32));
+    GBK_TRY(ensure_init());
+    hipStream_t st = stream();
+    void *ws, *out;
+    const size_t wb = mssp_ws_bytes(A->n, k);
+    GBK_TRY(scratch(SLOT_MSSSP, wb, &ws));
+    GBK_TRY(scratch(SLOT_MSBFS_OUT, sizeof(int32_t) * (size_t)k * A->n, &out));
+    GBK_TRY(mssp_run(A, s32.data(), k, ws, wb, (int32_t*)out, stats, st));
+    GBK_HIP(hipMemcpyAsync(dist, out, sizeof(int32_t) * (size_t)k * A->n, hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}

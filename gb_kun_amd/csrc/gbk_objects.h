@@ -31,6 +31,10 @@ struct DevCSR {
     void* carry = nullptr;  // 2*ntiles elements of the value type (tail, head)
     // row-length statistics (for kernel selection)
     int32_t max_row = 0;
+    // merge-path start coordinates of the SpMM lane-group tiles (built on
+    // first multi-vector use)
+    int64_t spmm_ntiles = 0;
+    int2* spmm_tiles = nullptr;
 };
 
 struct Workspace {
@@ -69,6 +73,7 @@ namespace gbk {
 // runtime.hip
 int ensure_init();
 hipStream_t stream();
+void set_thread_stream(hipStream_t s);
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
 int device_cu_count();

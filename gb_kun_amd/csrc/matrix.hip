@@ -21,6 +21,7 @@ static void free_csr(DevCSR& m) {
     dev_free(m.val);
     dev_free(m.tiles);
     dev_free(m.carry);
+    dev_free(m.spmm_tiles);
     m = DevCSR();
 }
 

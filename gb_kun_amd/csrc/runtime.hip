@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "gbk_objects.h"
+#include "batch.h"
 
 namespace gbk {
 
@@ -43,7 +44,7 @@ static hipStream_t g_stream = nullptr;
 static int g_cus = 0;
 
 struct ScratchSlot { void* p = nullptr; size_t bytes = 0; };
-static ScratchSlot g_scratch[8];
+static ScratchSlot g_scratch[16];
 
 int init_device(int device) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -74,7 +75,11 @@ int ensure_init() {
     return init_device(0);
 }
 
-hipStream_t stream() { return g_stream; }
+// A thread may route the library's launches to a stream of its own (the
+// query server runs one stream per worker thread).
+static thread_local hipStream_t t_stream = nullptr;
+hipStream_t stream() { return t_stream ? t_stream : g_stream; }
+void set_thread_stream(hipStream_t s) { t_stream = s; }
 int device_cu_count() { return g_cus; }
 
 int dev_alloc(void** p, size_t bytes) {
@@ -97,7 +102,7 @@ int scratch(int slot, size_t bytes, void** p) {
     ScratchSlot& s = g_scratch[slot];
     if (s.bytes < bytes) {
         if (s.p) {
-            GBK_HIP(hipStreamSynchronize(g_stream));
+            GBK_HIP(hipStreamSynchronize(stream()));
             dev_free(s.p);
             s.p = nullptr;
             s.bytes = 0;
@@ -195,8 +200,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* __re
     if (base <= n - 1 && n - 1 < base + SCAN_ITEMS) out[n] = (int32_t)run;
 }
 
-int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* total_host) {
-    hipStream_t st = stream();
+size_t scan_partial_bytes(int64_t n) { return sizeof(long long) * (size_t)((n + SCAN_TILE - 1) / SCAN_TILE + 1); }
+
+int exclusive_scan_i32_on(const int32_t* in, int32_t* out, int64_t n, int32_t* total_host, hipStream_t st,
+                          void* partial_ws, size_t ws_bytes) {
     if (n <= 0) {
         int32_t z = 0;
         GBK_HIP(hipMemcpyAsync(out, &z, sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -205,9 +212,8 @@ int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tota
         return GBK_SUCCESS;
     }
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    void* p;
-    GBK_TRY(scratch(0, sizeof(long long) * nb, &p));
-    long long* partial = (long long*)p;
+    if (ws_bytes < sizeof(long long) * (size_t)nb) return fail(GBK_INSUFFICIENT_SPACE, "scan workspace too small");
+    long long* partial = (long long*)partial_ws;
     // The per-tile partials must use the same element-to-tile map as the
     // apply pass: tile b owns [b*TILE, (b+1)*TILE).
     hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_THREADS), 0, st, in, n, partial);
@@ -221,6 +227,13 @@ int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tota
         GBK_HIP(hipStreamSynchronize(st));
     }
     return GBK_SUCCESS;
+}
+
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* total_host) {
+    void* p = nullptr;
+    const size_t b = scan_partial_bytes(n);
+    GBK_TRY(scratch(0, b, &p));
+    return exclusive_scan_i32_on(in, out, n, total_host, stream(), p, b);
 }
 
 template <typename T> int fill_typed(T* dst, int64_t n, T v);

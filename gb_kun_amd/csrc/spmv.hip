@@ -300,14 +300,16 @@ static int launch_rows(const DevCSR& m, const T* x, T* y, int64_t count, const i
 }
 
 template <typename T, int ADD, int MUL>
-static int launch_merge(const DevCSR& m, const T* x, T* y, int accum, const int32_t* tile_map = nullptr) {
+static int launch_merge(const DevCSR& m, const T* x, T* y, int accum, const int32_t* tile_map = nullptr,
+                        void* carry_ws = nullptr) {
     if (m.ntiles == 0) return GBK_SUCCESS;
     hipStream_t st = stream();
+    T* carry = (T*)(carry_ws ? carry_ws : m.carry);
     hipLaunchKernelGGL((k_spmv_merge<T, ADD, MUL>), dim3((unsigned)m.ntiles), dim3(MP_THREADS), 0, st, m.row_ptr, m.col,
-                       (const T*)m.val, x, y, (int32_t)m.nrows, m.tiles, (T*)m.carry, accum, tile_map);
+                       (const T*)m.val, x, y, (int32_t)m.nrows, m.tiles, carry, accum, tile_map);
     GBK_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_spmv_merge_fixup<T, ADD>), dim3((unsigned)((m.ntiles + 255) / 256)), dim3(256), 0, st,
-                       m.row_ptr, m.tiles, m.ntiles, (const T*)m.carry, y, accum);
+                       m.row_ptr, m.tiles, m.ntiles, (const T*)carry, y, accum);
     GBK_LAUNCH_CHECK();
     return GBK_SUCCESS;
 }
@@ -320,6 +322,22 @@ int spmv_csr_unmasked(const DevCSR& m, int type, const void* x, void* y, int add
             if (order_sensitive(ADD))
                 return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, m.nrows, nullptr, nullptr, 0, 1, accum, 1);
             return launch_merge<T, ADD, MUL>(m, (const T*)x, (T*)y, accum);
+        });
+    });
+}
+
+// Merge-path SpMV with a caller-owned carry buffer (merge_carry_bytes(m)
+// bytes), so that SpMVs on one matrix may run concurrently on several streams.
+size_t merge_carry_bytes(const DevCSR& m) { return 2 * sizeof(double) * (size_t)(m.ntiles > 0 ? m.ntiles : 1); }
+
+int spmv_merge_ws(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum, void* carry) {
+    return dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
+            if (order_sensitive(ADD))
+                return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, m.nrows, nullptr, nullptr, 0, 1, accum, 1);
+            return launch_merge<T, ADD, MUL>(m, (const T*)x, (T*)y, accum, nullptr, carry);
         });
     });
 }

@@ -350,6 +350,43 @@ int gbk_cc(gbk_matrix A, int32_t* labels, int64_t* stats);
 int gbk_pagerank(gbk_matrix A, float* ranks, int itermax, double damping, double tol, int* iters);
 
 /* ------------------------------------------------------------------------ */
+/* Batched (multi-vector) operations.  The reference has no batched form: each
+ * is the k-fold of one reference call, computed in one pass over the matrix. */
+/* ------------------------------------------------------------------------ */
+
+/* Y = accum(Y, A * X) for k right-hand sides at once (1 <= k <= 64): X and Y
+ * are device arrays of n*k elements of the matrix type, row-major (the k
+ * values of vertex v at v*k .. v*k+k-1).  Column q of Y equals
+ * gbk_spmv_device on column q of X (SpMV_all_active_diff_vectors,
+ * spmv_csr.h:341-384, k times).  The monoid must be commutative. */
+int gbk_spmm_device(gbk_matrix A, int transposed, const void* X, void* Y, int k,
+                    gbk_semiring op, int accum);
+
+/* Multi-source BFS: nsrc <= 64 searches in one traversal (bfs_blast,
+ * algorithms/bfs/bfs.hpp:94-142, once per source).  levels: nsrc*n int32,
+ * source-major, 1-based, 0 = unreached.  stats (may be NULL, >= 5 entries):
+ * [0] levels, [1] push steps, [2] pull steps, [3] frontier edges,
+ * [4] (vertex, source) pairs reached. */
+int gbk_msbfs(gbk_matrix A, const int64_t* sources, int nsrc, int32_t* levels, int64_t* stats);
+/* Same, writing the levels to a device array of nsrc*n int32. */
+int gbk_msbfs_device(gbk_matrix A, const int64_t* sources, int nsrc, int32_t* levels_dev, int64_t* stats);
+
+/* Personalized PageRank for k <= 64 sources at once on an FP32 transition
+ * matrix (entry (v,u) = 1/outdeg(u)): r_q = d A r_q + ((1-d) + d sink_q) e_q,
+ * the per-source form of LAGraph_page_rank_sinks (algorithms/pr/pr.hpp:20-153).
+ * Column q stops when its L1 change is below tol (tol > 0) or after itermax
+ * iterations.  ranks: k*n floats (host), source-major; iters[q]: iterations
+ * of column q. */
+int gbk_ppr_multi(gbk_matrix A, const int64_t* sources, int k, float* ranks, int itermax,
+                  double damping, double tol, int* iters);
+
+/* min.plus SSSP for k <= 64 sources at once on an INT32 weight matrix
+ * (sssp_bellman_ford_blast, algorithms/sssp/sssp_blast.hpp:8-72, once per
+ * source).  dist: k*n int32 (host), source-major, INT32_MAX = unreachable.
+ * stats (may be NULL): [0] rounds, [1] edge relaxations. */
+int gbk_sssp_multi(gbk_matrix A, const int64_t* sources, int k, int32_t* dist, int64_t* stats);
+
+/* ------------------------------------------------------------------------ */
 /* Seeded graph generation (host) — replaces the unseeded
  * GraphGenerationAPI::RMAT / random_uniform
  * (src/helpers/graph_generation/graph_generation.hpp:37-190).

@@ -545,4 +545,45 @@ int orc_pagerank_f64_q(int64_t n, const int64_t* rp, const int32_t* col, int ite
     return it;
 }
 
+/* Personalized PageRank in FP64 on an explicit transition matrix:
+ *   r' = d * A r + ((1 - d) + d * sink(r)) * p,   sink(r) = sum of r over the
+ *   vertices whose column of A is empty,
+ * starting from r = p, until the L1 change is below tol (tol > 0) or after
+ * itermax iterations; returns the iterations run.  With p = 1/n and
+ * A = 1/coldeg this is LAGraph_page_rank_sinks (algorithms/pr/pr.hpp:20-153,
+ * the same recurrence as orc_pagerank_f64); with p = e_s it is the
+ * personalized form the batched driver computes. */
+int orc_ppr_f64(int64_t n, const int64_t* rp, const int32_t* col, const float* val, const double* p, int itermax,
+                double damping, double tol, double* r) {
+    int64_t nnz = rp[n];
+    char* has = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
+    for (int64_t j = 0; j < nnz; ++j) has[col[j]] = 1;
+    double* t = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) r[i] = p[i];
+    int it;
+    for (it = 0; it < itermax; ++it) {
+        double sink = 0.0;
+        for (int64_t i = 0; i < n; ++i)
+            if (!has[i]) sink += r[i];
+        const double tele = (1.0 - damping) + damping * sink;
+        double diff = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int64_t j = rp[i]; j < rp[i + 1]; ++j) s += (double)val[j] * r[col[j]];
+            t[i] = damping * s + tele * p[i];
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            diff += fabs(t[i] - r[i]);
+            r[i] = t[i];
+        }
+        if (tol > 0 && diff < tol) {
+            ++it;
+            break;
+        }
+    }
+    free(has);
+    free(t);
+    return it;
+}
+
 int orc_version(void) { return 1; }

@@ -177,6 +177,21 @@ _ref = None
 _ref_cc = None
 
 
+def ppr_f64(rp, col, val, source=None, p=None, itermax=100, damping=0.85, tol=0.0):
+    """FP64 personalized PageRank (orc_ppr_f64): teleport to `source` (one-hot) or to the vector p."""
+    rp, col = _csr(rp, col)
+    n = len(rp) - 1
+    val = np.ascontiguousarray(val, dtype=np.float32)
+    if p is None:
+        p = np.zeros(n, dtype=np.float64)
+        p[source] = 1.0
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    r = np.empty(n, dtype=np.float64)
+    it = lib().orc_ppr_f64(C.c_int64(n), _p(rp), _p(col), _p(val), _p(p), C.c_int(itermax), C.c_double(damping),
+                           C.c_double(tol), _p(r))
+    return r, it
+
+
 def have_ref():
     return os.path.exists(os.path.join(REF_DIR, "libgbkref.so"))
 

@@ -1,0 +1,129 @@
+"""GPU parity of the batched (multi-vector) path.
+
+* SpMM: column q of A (+.*) X equals the single-vector SpMV of column q —
+  bit-exact for integer semirings, within 1e-5 relative of the FP64 oracle
+  for fp32 plus.times.
+* Multi-source BFS: levels of every source equal the oracle's queue BFS
+  (bit-exact), 1-based with 0 = unreached.
+* Multi-source SSSP: distances equal the oracle's Dijkstra (bit-exact).
+* Personalized PageRank: every column within 1e-6 L1 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from tests.graphs import special_graphs
+
+pytestmark = pytest.mark.gpu
+
+INT_MAX = np.iinfo(np.int32).max
+
+
+def _sources(rp, k, seed):
+    deg = np.diff(rp)
+    cand = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(seed)
+    return rng.choice(cand, size=min(k, len(cand)), replace=False)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 17, 32, 64])
+def test_spmm_int_exact(gbk, oracle, k):
+    torch = _torch()
+    graphs = {"rmat13": gbk.gen_rmat(13, 16, seed=81), "er12": gbk.gen_uniform(12, 8, seed=82)}
+    graphs.update({k2: v for k2, v in special_graphs().items() if k2 in ("hub_row", "tile_edges", "empty_run")})
+    for gname, (rp, col) in graphs.items():
+        n = len(rp) - 1
+        vals = (1 + (gbk.uniform_f32(83, len(col)) * 255).astype(np.int64)).astype(np.int32)
+        A = gbk.Matrix.from_csr(rp, col, vals)
+        X = (gbk.uniform_f32(84, n * k).reshape(n, k) * 1000).astype(np.int32)
+        for sr in ("PlusMultiplies", "MinimumPlus"):
+            add, mul = gbk.SEMIRINGS[sr]
+            Xd = torch.from_numpy(X).cuda()
+            Yd = torch.zeros((n, k), dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()
+            gbk.spmm_device(A, Xd.data_ptr(), Yd.data_ptr(), k, sr)
+            gbk.synchronize()
+            Y = Yd.cpu().numpy()
+            for q in sorted({0, k // 2, k - 1}):
+                want = oracle.spmv(rp, col, vals, X[:, q].copy(), np.zeros(n, np.int32), add, mul)
+                assert np.array_equal(Y[:, q], want), f"{gname} {sr} k={k} column {q}"
+
+
+@pytest.mark.parametrize("k", [2, 32])
+def test_spmm_fp32_and_accum(gbk, oracle, k):
+    torch = _torch()
+    rp, col = gbk.gen_rmat(14, 16, seed=85)
+    n = len(rp) - 1
+    vals = gbk.uniform_f32(86, len(col))
+    A = gbk.Matrix.from_csr(rp, col, vals)
+    X = gbk.uniform_f32(87, n * k).reshape(n, k)
+    Y0 = gbk.uniform_f32(88, n * k).reshape(n, k)
+    Xd = torch.from_numpy(X).cuda()
+    Yd = torch.from_numpy(Y0.copy()).cuda()
+    torch.cuda.synchronize()
+    gbk.spmm_device(A, Xd.data_ptr(), Yd.data_ptr(), k, "PlusMultiplies", accum=gbk.ACC_PLUS)
+    gbk.synchronize()
+    Y = Yd.cpu().numpy()
+    for q in range(k):
+        y64, absum = oracle.spmv_f64(rp, col, vals, X[:, q].copy())
+        err = np.abs(Y[:, q].astype(np.float64) - (Y0[:, q].astype(np.float64) + y64))
+        assert np.all(err <= 1e-5 * (absum + np.abs(Y0[:, q])) + 1e-30), f"column {q}"
+
+
+def test_spmm_rejects_bad_k(gbk):
+    rp, col = gbk.gen_rmat(8, 8, seed=1)
+    A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    with pytest.raises(gbk.GbkError):
+        gbk.spmm_device(A, 1 << 20, 2 << 20, 65)  # rejected before any device access
+
+
+@pytest.mark.parametrize("scale,sym,nsrc", [(12, True, 64), (14, False, 64), (15, True, 17), (10, True, 1)])
+def test_msbfs_vs_oracle(gbk, oracle, scale, sym, nsrc):
+    rp, col = gbk.gen_rmat(scale, 16, seed=90 + scale, symmetric=sym)
+    A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    src = _sources(rp, nsrc, 91)
+    lv, st = gbk.msbfs(A, src)
+    assert st[0] >= 1
+    for q, s in enumerate(src):
+        want = oracle.bfs(rp, col, int(s))
+        assert np.array_equal(lv[q], want), f"source {s} (slot {q})"
+
+
+def test_msbfs_duplicate_and_isolated_sources(gbk, oracle):
+    rp, col = gbk.gen_rmat(11, 8, seed=95, symmetric=True)
+    A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    iso = np.flatnonzero(np.diff(rp) == 0)
+    src = [int(_sources(rp, 1, 96)[0])] * 3 + ([int(iso[0])] if len(iso) else [])
+    lv, _ = gbk.msbfs(A, src)
+    for q, s in enumerate(src):
+        assert np.array_equal(lv[q], oracle.bfs(rp, col, s))
+
+
+@pytest.mark.parametrize("scale,k", [(12, 32), (13, 7), (11, 64)])
+def test_sssp_multi_vs_dijkstra(gbk, oracle, scale, k):
+    rp, col = gbk.gen_rmat(scale, 16, seed=100 + scale, symmetric=True)
+    w = gbk.sym_weights_i32(rp, col, 101, 1, 255)
+    A = gbk.Matrix.from_csr(rp, col, w)
+    src = _sources(rp, k, 102)
+    d, st = gbk.sssp_multi(A, src)
+    for q, s in enumerate(src):
+        want = oracle.dijkstra_i32(rp, col, w, int(s))
+        assert np.array_equal(d[q], want), f"source {s}"
+
+
+@pytest.mark.parametrize("scale,k", [(12, 32), (13, 1), (11, 5)])
+def test_ppr_multi_vs_fp64(gbk, oracle, scale, k):
+    rp, col = gbk.gen_rmat(scale, 16, seed=110 + scale)
+    n = len(rp) - 1
+    vals = gbk.inv_coldeg_f32(n, col)
+    A = gbk.Matrix.from_csr(rp, col, vals)
+    src = _sources(rp, k, 111)
+    r, its = gbk.ppr_multi(A, src, itermax=200, tol=1e-7)
+    for q, s in enumerate(src):
+        want, _ = oracle.ppr_f64(rp, col, vals, source=int(s), itermax=int(its[q]))
+        l1 = np.abs(r[q].astype(np.float64) - want).sum()
+        assert l1 <= 1e-6, f"source {s}: L1 {l1:.2e} after {its[q]} iterations"

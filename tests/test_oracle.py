@@ -231,3 +231,37 @@ def test_pagerank_fixed_point(oracle):
     assert abs(r.sum() - 1.0) < 1e-9
     r2, _ = oracle.pagerank_f64(rp, col, itermax=it + 5, tol=0.0)
     assert np.abs(r2 - r).sum() < 1e-11
+
+
+def _dense_ppr(rp, col, val, p, damping):
+    """Closed form of the personalized PageRank fixed point on a small graph:
+    (I - d A - d p s^T) r = (1 - d) p, s = indicator of the empty columns."""
+    n = len(rp) - 1
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in range(rp[i], rp[i + 1]):
+            A[i, col[j]] += float(val[j])
+    s = np.zeros(n)
+    s[np.setdiff1d(np.arange(n), np.unique(col))] = 1.0
+    M = np.eye(n) - damping * A - damping * np.outer(p, s)
+    return np.linalg.solve(M, (1.0 - damping) * p)
+
+
+def test_ppr_closed_form(oracle, gbk):
+    """The FP64 personalized PageRank converges to the linear-system solution,
+    sums to 1, and with a uniform teleport vector equals the reference's
+    PageRank with sinks (orc_pagerank_f64, an independent restatement)."""
+    rp, col = _rand_csr(11, 120, 500)
+    n = len(rp) - 1
+    val = gbk.inv_coldeg_f32(n, col)
+    for src in (0, 7, 63):
+        p = np.zeros(n)
+        p[src] = 1.0
+        r, it = oracle.ppr_f64(rp, col, val, source=src, itermax=2000, tol=1e-14)
+        want = _dense_ppr(rp, col, val, p, 0.85)
+        assert np.abs(r - want).sum() < 1e-10, src
+        assert abs(r.sum() - 1.0) < 1e-6  # columns of fp32 1/coldeg sum to 1 within rounding
+    # uniform teleport vector: the global PageRank with sinks
+    r_u, _ = oracle.ppr_f64(rp, col, val, p=np.full(n, 1.0 / n), itermax=300)
+    r_g, _ = oracle.pagerank_f64(rp, col, itermax=300)
+    assert np.abs(r_u - r_g).sum() < 1e-6  # fp32 1/coldeg values vs exact division
