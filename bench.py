@@ -29,6 +29,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GTEPS & HBM GB/s (% peak), semiring SpMV/BFS RMAT-22; batched/served q/s, p99"
 HBM_PEAK_GBS = 8000.0
+KERNELS = {
+    "csr": "k_spmv_merge + k_spmv_merge_fixup",
+    "csr_seg": "k_spmv_merge (XCD-mapped segment tiles) + k_spmv_merge_fixup + k_seg_merge",
+    "lav": "k_gather + k_spmv_merge (XCD-mapped segment tiles) + k_spmv_merge_fixup + k_seg_merge",
+    "sell_c": "k_sell_c + long-row k_spmv_merge + k_spmv_merge_fixup",
+}
 
 
 def log(*a):
@@ -42,12 +48,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--format", default="csr", choices=["csr", "sell_c", "csr_seg", "lav"])
+    ap.add_argument("--format", default="auto", choices=["auto", "csr", "sell_c", "csr_seg", "lav"])
     ap.add_argument("--graph", default="rmat", choices=["rmat", "er"])
     ap.add_argument("--sell-c", type=int, default=0)
     ap.add_argument("--sigma", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the timed loop, copy this many MiB device-to-device on the library stream "
+                         "(a kernel of known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -55,8 +64,8 @@ def main():
               f"(the multi-GPU row sharding is design-only).", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import gb_kun_amd as g
+    from gb_kun_amd.device import DeviceArray, Event
 
     t0 = time.time()
     if args.graph == "rmat":
@@ -71,47 +80,50 @@ def main():
         f"empty rows={int(np.count_nonzero(np.diff(rp) == 0))} max row={int(np.diff(rp).max())} "
         f"({time.time() - t0:.1f} s)")
 
-    fmt = {"csr": g.FMT_CSR, "sell_c": g.FMT_SELL_C, "csr_seg": g.FMT_CSR_SEG, "lav": g.FMT_LAV}[args.format]
     t0 = time.time()
-    A = g.Matrix.from_csr(rp, col, vals, fmt=fmt, sell_c=args.sell_c, sigma=args.sigma)
+    tuned = None
+    if args.format == "auto":
+        A = g.Matrix.from_csr(rp, col, vals)
+        f, c, times = A.autotune(trials=5)
+        fmt_name = {g.FMT_CSR: "csr", g.FMT_CSR_SEG: "csr_seg", g.FMT_LAV: "lav"}.get(f, f"sell_c{c}")
+        tuned = {k: round(v, 1) for k, v in times.items()}
+        log(f"autotuner (us per SpMV): {tuned} -> {fmt_name}")
+    else:
+        fmt = {"csr": g.FMT_CSR, "sell_c": g.FMT_SELL_C, "csr_seg": g.FMT_CSR_SEG, "lav": g.FMT_LAV}[args.format]
+        A = g.Matrix.from_csr(rp, col, vals, fmt=fmt, sell_c=args.sell_c, sigma=args.sigma)
+        fmt_name = args.format
     log(f"device matrix built in {time.time() - t0:.1f} s, {A.device_bytes() / 2**20:.0f} MiB of HBM")
     cu, arch = g.device_info()
 
-    # The kernels run on the library's own stream; the timing events are
-    # recorded on that same stream.
-    stream = torch.cuda.ExternalStream(int(g.lib.gbk_get_stream()))
-    x = torch.from_numpy(x_np).cuda()
-    y = torch.zeros(n, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
+    x = DeviceArray.from_numpy(x_np)
+    y = DeviceArray.zeros(n, np.float32)
 
     def step():
-        g.spmv_device(A, x.data_ptr(), y.data_ptr())
+        g.spmv_device(A, x.ptr, y.ptr)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    g.synchronize()
 
     # correctness spot check of the timed configuration against FP64
     from oracle import oracle as O
-    y_np = y.cpu().numpy()
+    y_np = y.numpy()
     rows = np.random.default_rng(0).choice(n, size=min(n, 20000), replace=False)
     want, absum = O.spmv_f64(rp, col, vals, x_np)
     err = np.max(np.abs(y_np[rows].astype(np.float64) - want[rows]) / np.maximum(absum[rows], 1e-30))
     log(f"spot check vs FP64 on {len(rows)} rows: max rel err {err:.2e}")
 
-    torch.cuda.synchronize()
+    # HIP events on the library stream, where the kernels run
     g.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0, ev1 = Event(), Event()
     w0 = time.perf_counter()
-    ev0.record(stream)
+    ev0.record()
     for _ in range(args.steps):
         step()
-    ev1.record(stream)
+    ev1.record()
     ev1.synchronize()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    ms = ev0.elapsed_time(ev1) / args.steps
+    ms = ev0.elapsed_ms(ev1) / args.steps
 
     algo_bytes = 4 * (n + 1) + 8 * nnz + 4 * n + 4 * n
     ref_bytes = 16 * nnz
@@ -121,10 +133,18 @@ def main():
         f"{gteps:.1f} GTEPS  wall {wall * 1e3 / args.steps:.1f} us/step")
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_spmv_{args.format}_rmat{args.scale}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_spmv_{fmt_name}_{args.graph}{args.scale}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    if args.copy_probe > 0:
+        from gb_kun_amd.device import copy_d2d
+        a = DeviceArray(args.copy_probe << 18, np.float32)
+        b = DeviceArray(args.copy_probe << 18, np.float32)
+        copy_d2d(b, a)
+        g.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB device-to-device")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -145,14 +165,14 @@ def main():
         "data": "synthetic (seeded R-MAT, values 1/coldeg)",
         "config": {"workload": f"plus.times fp32 SpMV, {'R-MAT' if args.graph == 'rmat' else 'Erdos-Renyi'} scale "
                                f"{args.scale}, edge factor {args.edge_factor}, "
-                               f"{args.format} format", "n": n, "nnz": nnz, "format": args.format,
+                               f"{fmt_name} format", "n": n, "nnz": nnz, "format": fmt_name, "autotuner_us": tuned,
                    "parallelism": "single GPU"},
         "gteps": round(gteps, 3),
         "ref_accounting_gbs": round(ref_bytes / (ms * 1e-3) / 1e9, 2),
         "device": {"arch": arch, "cus": cu},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmv_merge + k_spmv_merge_fixup (one SpMV)",
+                     "kernel": KERNELS.get(fmt_name.rstrip("0123456789"), "SpMV"),
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
     }

@@ -94,6 +94,7 @@ _sig("gbk_matrix_export", C.c_int, _vp, C.c_int, _pi64, _pi32, _vp)
 _sig("gbk_matrix_degrees", C.c_int, _vp, C.c_int, _pi64)
 _sig("gbk_matrix_device_bytes", C.c_int, _vp, _pi64)
 _sig("gbk_matrix_format_stats", C.c_int, _vp, C.c_int, _pi64)
+_sig("gbk_matrix_autotune", C.c_int, _vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double))
 _sig("gbk_vector_new", C.c_int, C.POINTER(_vp), C.c_int, _i64)
 _sig("gbk_vector_free", C.c_int, _vp)
 _sig("gbk_vector_nvals", C.c_int, _vp, _pi64)
@@ -297,6 +298,15 @@ class Matrix:
         check(lib.gbk_matrix_format_stats(self.h, int(transposed), _ptr(st, C.c_int64)))
         return tuple(int(v) for v in st)
 
+    def autotune(self, trials=5):
+        """Times CSR, segmented CSR, LAV, SELL-C (C=64, C=32) and keeps the fastest.
+        Returns (format, C, {name: microseconds})."""
+        f, c = C.c_int(0), C.c_int(0)
+        t = (C.c_double * 5)()
+        check(lib.gbk_matrix_autotune(self.h, trials, C.byref(f), C.byref(c), t))
+        names = ["csr", "csr_seg", "lav", "sell_c64", "sell_c32"]
+        return f.value, c.value, dict(zip(names, list(t)))
+
     def free(self):
         if self.h:
             lib.gbk_matrix_free(self.h)
@@ -398,7 +408,7 @@ def vxm(w, mask, accum, semiring, u, A, desc_mask=DEFAULT):
 
 
 def spmv_device(A, x_ptr, y_ptr, semiring=(ADD_PLUS, MUL_TIMES), accum=ACC_NONE, transposed=False):
-    """y = accum(y, A (+.*) x) on raw device pointers (e.g. torch tensor data_ptr())."""
+    """y = accum(y, A (+.*) x) on raw device pointers (e.g. gb_kun_amd.device.DeviceArray.ptr)."""
     add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
     check(lib.gbk_spmv_device(A.h, int(transposed), _vp(x_ptr), _vp(y_ptr), Semiring(add, mul), accum))
 

@@ -10,7 +10,8 @@
 // entries is "problematic" (build.hpp:147-155): it is not stored in the
 // SELL-C arrays, and its rows are listed for a row-split CSR kernel (the
 // reference's fallback loop for them returns the identity, a defect not
-// reproduced here).
+// reproduced here).  Rows longer than long_thr leave their chunks the same
+// way, so that no lane of the chunk kernel walks a power-law hub row alone.
 //
 // Segmented CSR — replaces MatrixSegmentedCSR::build
 // (seg_csr/build.hpp:24-169): columns are cut into segments of seg_cols;
@@ -36,7 +37,7 @@
 namespace gbk_host {
 
 void build_sell_c(int64_t n, const int32_t* rp, const int32_t* col, const void* val, size_t vs, int C, int sigma,
-                  SellCHost& out) {
+                  int64_t long_thr, SellCHost& out) {
     out = SellCHost();
     out.C = C;
     out.sigma = sigma;
@@ -68,10 +69,11 @@ void build_sell_c(int64_t n, const int32_t* rp, const int32_t* col, const void* 
             if (p < n) {
                 int32_t row = perm[p];
                 int64_t L = len(row);
-                mx = std::max(mx, L);
-                tot += L;
                 out.row_perm[p] = row;
                 out.row_len[p] = (int32_t)L;
+                if (L > long_thr) continue;  // a long row leaves the chunk (see below)
+                mx = std::max(mx, L);
+                tot += L;
             }
         }
         if (mx * C >= 2 * tot && tot > 1024) {
@@ -83,13 +85,25 @@ void build_sell_c(int64_t n, const int32_t* rp, const int32_t* col, const void* 
     }
     for (int64_t c = 0; c < nchunks; ++c) {
         out.chunk_ptr[c + 1] = out.chunk_ptr[c] + (int64_t)out.chunk_len[c] * C;
+        if (!problematic[c]) {
+            // long rows are served by the row-split kernel: their slots stay empty
+            for (int r = 0; r < C; ++r) {
+                int64_t p = c * C + r;
+                if (p < n && out.row_len[p] > long_thr) {
+                    out.long_rows.push_back(out.row_perm[p]);
+                    out.row_len[p] = 0;
+                    out.row_perm[p] = -1;
+                }
+            }
+        }
         if (problematic[c]) {
             for (int r = 0; r < C; ++r) {
                 int64_t p = c * C + r;
                 if (p >= n) continue;
-                // every row of the chunk (empty ones too) goes to the row kernel;
+                // every row of the chunk (empty ones too) goes to a row kernel;
                 // the chunk kernel skips positions whose row is -1
-                out.problem_rows.push_back(out.row_perm[p]);
+                if (out.row_len[p] > long_thr) out.long_rows.push_back(out.row_perm[p]);
+                else out.problem_rows.push_back(out.row_perm[p]);
                 out.row_len[p] = 0;
             }
         }

@@ -25,9 +25,9 @@
 namespace gbk {
 
 int spmv_merge_mapped(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
-                      const int32_t* tile_map);
+                      const int32_t* tile_map, const int32_t* row_map = nullptr);
 int spmv_rows_list(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
-                   const int32_t* rows, int64_t count);
+                   const int32_t* rows, int64_t count, int group = 0);
 
 struct SellC {
     int C = 64, sigma = 512;
@@ -38,7 +38,10 @@ struct SellC {
     int32_t* row_len = nullptr;
     int32_t* col = nullptr;
     void* val = nullptr;
-    int32_t* problem_rows = nullptr;
+    int32_t* problem_rows = nullptr;  // short rows of problematic chunks (sequential fold)
+    int64_t nlong = 0;
+    int32_t* long_rows = nullptr;     // rows longer than 4C entries, ascending
+    DevCSR lng;                       // CSR of the long rows (row i = long_rows[i])
 };
 
 struct SegCSR {
@@ -92,6 +95,12 @@ static void free_sell(SellC* s) {
     dev_free(s->col);
     dev_free(s->val);
     dev_free(s->problem_rows);
+    dev_free(s->long_rows);
+    dev_free(s->lng.row_ptr);
+    dev_free(s->lng.col);
+    dev_free(s->lng.val);
+    dev_free(s->lng.tiles);
+    dev_free(s->lng.carry);
     delete s;
 }
 
@@ -100,13 +109,16 @@ static int build_sell(const DevCSR& m, int type, int C, int sigma, SellC** out) 
     std::vector<char> val;
     GBK_TRY(download_csr(m, type, rp, col, val));
     gbk_host::SellCHost h;
-    gbk_host::build_sell_c(m.nrows, rp.data(), col.data(), val.data(), type_size(type), C, sigma, h);
+    // rows longer than 4 C entries go to the merge-path kernel
+    gbk_host::build_sell_c(m.nrows, rp.data(), col.data(), val.data(), type_size(type), C, sigma, 4 * (int64_t)C, h);
+    std::sort(h.long_rows.begin(), h.long_rows.end());
     SellC* s = new SellC();
     s->C = C;
     s->sigma = sigma;
     s->nchunks = h.nchunks;
     s->padded = h.padded;
     s->nproblem = (int64_t)h.problem_rows.size();
+    s->nlong = (int64_t)h.long_rows.size();
     int rc = upload(&s->chunk_ptr, h.chunk_ptr);
     if (rc == GBK_SUCCESS) rc = upload(&s->chunk_len, h.chunk_len);
     if (rc == GBK_SUCCESS) rc = upload(&s->row_perm, h.row_perm);
@@ -114,6 +126,32 @@ static int build_sell(const DevCSR& m, int type, int C, int sigma, SellC** out) 
     if (rc == GBK_SUCCESS) rc = upload(&s->col, h.col);
     if (rc == GBK_SUCCESS) rc = upload((char**)&s->val, h.val);
     if (rc == GBK_SUCCESS) rc = upload(&s->problem_rows, h.problem_rows);
+    if (rc == GBK_SUCCESS) rc = upload(&s->long_rows, h.long_rows);
+    if (rc == GBK_SUCCESS && !h.long_rows.empty()) {
+        // CSR of the long rows, in ascending row order
+        const size_t vs = type_size(type);
+        const int64_t nl = (int64_t)h.long_rows.size();
+        std::vector<int32_t> lrp(nl + 1, 0), lcol;
+        std::vector<char> lval;
+        for (int64_t i = 0; i < nl; ++i) {
+            const int32_t r = h.long_rows[i];
+            lrp[i + 1] = lrp[i] + (rp[r + 1] - rp[r]);
+        }
+        lcol.resize(lrp[nl]);
+        lval.resize((size_t)lrp[nl] * vs);
+        for (int64_t i = 0; i < nl; ++i) {
+            const int32_t r = h.long_rows[i];
+            std::copy(col.begin() + rp[r], col.begin() + rp[r + 1], lcol.begin() + lrp[i]);
+            std::copy(val.begin() + (size_t)rp[r] * vs, val.begin() + (size_t)rp[r + 1] * vs, lval.begin() + (size_t)lrp[i] * vs);
+        }
+        s->lng.nrows = nl;
+        s->lng.ncols = m.ncols;
+        s->lng.nnz = lrp[nl];
+        rc = upload(&s->lng.row_ptr, lrp);
+        if (rc == GBK_SUCCESS) rc = upload(&s->lng.col, lcol);
+        if (rc == GBK_SUCCESS) rc = upload((char**)&s->lng.val, lval);
+        if (rc == GBK_SUCCESS) rc = build_merge_tiles(s->lng, type);
+    }
     if (rc != GBK_SUCCESS) {
         free_sell(s);
         return rc;
@@ -137,11 +175,32 @@ __global__ __launch_bounds__(256) void k_sell_c(const int64_t* __restrict__ chun
     const int32_t L = row_len[p];
     const int64_t base = chunk_ptr[c] + r;
     T acc = add_identity<ADD, T>();
-    for (int32_t j = 0; j < L; ++j) {
+    int32_t j = 0;
+    // four slices per step, all loads issued before the fold (the fold
+    // itself stays in storage order: the reference's left fold)
+    for (; j + 4 <= L; j += 4) {
+        int32_t cc[4];
+        T vv[4], xx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t e = base + (int64_t)(j + u) * C;
+            cc[u] = __builtin_nontemporal_load(col + e);
+            vv[u] = __builtin_nontemporal_load(val + e);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xx[u] = x[cc[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = add_op<ADD, T>(acc, mul_op<MUL, T>(vv[u], xx[u]));
+    }
+    for (; j < L; ++j) {
         const int64_t e = base + (int64_t)j * C;
         acc = add_op<ADD, T>(acc, mul_op<MUL, T>(__builtin_nontemporal_load(val + e), x[__builtin_nontemporal_load(col + e)]));
     }
     y[row] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[row], acc);
+}
+
+static bool order_sensitive_monoid(int add) {
+    return add == GBK_ADD_LESS || add == GBK_ADD_FIRST || add == GBK_ADD_SECOND;
 }
 
 static int sell_spmv(const SellC* s, const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum) {
@@ -159,7 +218,15 @@ static int sell_spmv(const SellC* s, const DevCSR& m, int type, const void* x, v
             return GBK_SUCCESS;
         });
     }));
-    if (s->nproblem > 0) GBK_TRY(spmv_rows_list(m, type, x, y, add, mul, accum, s->problem_rows, s->nproblem));
+    // short rows of problematic chunks: one lane per row, the left fold
+    if (s->nproblem > 0) GBK_TRY(spmv_rows_list(m, type, x, y, add, mul, accum, s->problem_rows, s->nproblem, 1));
+    // long rows: merge path over their own CSR (order-sensitive monoids fold sequentially)
+    if (s->nlong > 0) {
+        if (order_sensitive_monoid(add))
+            GBK_TRY(spmv_rows_list(m, type, x, y, add, mul, accum, s->long_rows, s->nlong, 1));
+        else
+            GBK_TRY(spmv_merge_mapped(s->lng, type, x, y, add, mul, accum, nullptr, s->long_rows));
+    }
     return GBK_SUCCESS;
 }
 
@@ -395,7 +462,8 @@ int64_t format_bytes(gbk_matrix A) {
     int64_t b = 0;
     for (int t = 0; t < 2; ++t) {
         if (const SellC* s = A->sell[t])
-            b += (int64_t)(s->padded * (4 + vs) + s->nchunks * (8 + 4 + 8 * s->C) + s->nproblem * 4);
+            b += (int64_t)(s->padded * (4 + vs) + s->nchunks * (8 + 4 + 8 * s->C) + (s->nproblem + s->nlong) * 4 +
+                           s->lng.nnz * (4 + vs) + s->lng.nrows * 4);
         const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
         if (sg) b += (int64_t)(sg->npairs * (4 + 4 + vs + 4) + sg->v.nnz * (4 + vs) + sg->n * 4 + sg->v.ntiles * 28);
         if (A->lav[t]) b += A->lav[t]->seg.n * (4 + vs);
@@ -407,6 +475,76 @@ int64_t format_bytes(gbk_matrix A) {
 
 using namespace gbk;
 
+// Per-matrix format autotuner: builds each candidate representation, times
+// `trials` plus.times SpMVs (mxv over rows) with HIP events on the library
+// stream, keeps the fastest.  Candidates: CSR, segmented CSR, LAV,
+// SELL-C-sigma with C = 64 and C = 32 (sigma = 512).
+extern "C" int gbk_matrix_autotune(gbk_matrix A, int trials, int* chosen_format, int* chosen_c, double* times_us) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (trials < 1) trials = 3;
+    GBK_TRY(ensure_init());
+    struct Cand { int fmt, C; };
+    const Cand cands[5] = {{GBK_FMT_CSR, 0}, {GBK_FMT_CSR_SEG, 0}, {GBK_FMT_LAV, 0}, {GBK_FMT_SELL_C, 64},
+                           {GBK_FMT_SELL_C, 32}};
+    const int64_t n = A->n;
+    const size_t vs = type_size(A->type);
+    void *x = nullptr, *y = nullptr;
+    GBK_TRY(dev_alloc(&x, vs * (n > 0 ? n : 1)));
+    int rc = dev_alloc(&y, vs * (n > 0 ? n : 1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == GBK_SUCCESS && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(GBK_PANIC, "hipEventCreate failed");
+    if (rc == GBK_SUCCESS) rc = fill_value(A->type, x, n, 1.0);
+    const int add = A->type == GBK_BOOL ? GBK_ADD_LOR : GBK_ADD_PLUS;
+    const int mul = A->type == GBK_BOOL ? GBK_MUL_LAND : GBK_MUL_TIMES;
+    double best = -1.0;
+    int best_i = 0;
+    for (int i = 0; i < 5 && rc == GBK_SUCCESS; ++i) {
+        A->format = cands[i].fmt;
+        if (cands[i].C) {
+            A->sell_c = cands[i].C;
+            A->sigma = 512;
+        }
+        rc = format_build(A);
+        if (rc != GBK_SUCCESS) break;
+        rc = spmv_unmasked(A, 0, x, y, add, mul, GBK_ACC_NONE);  // warm-up
+        if (rc != GBK_SUCCESS) break;
+        (void)hipEventRecord(e0, stream());
+        for (int t = 0; t < trials && rc == GBK_SUCCESS; ++t) rc = spmv_unmasked(A, 0, x, y, add, mul, GBK_ACC_NONE);
+        (void)hipEventRecord(e1, stream());
+        if (rc != GBK_SUCCESS) break;
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(GBK_PANIC, "autotune timing failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double us = 1000.0 * ms / trials;
+        if (times_us) times_us[i] = us;
+        if (best < 0 || us < best) {
+            best = us;
+            best_i = i;
+        }
+    }
+    if (rc == GBK_SUCCESS) {
+        A->format = cands[best_i].fmt;
+        if (cands[best_i].C) {
+            A->sell_c = cands[best_i].C;
+            A->sigma = 512;
+        }
+        rc = format_build(A);
+        if (chosen_format) *chosen_format = cands[best_i].fmt;
+        if (chosen_c) *chosen_c = cands[best_i].C;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(stream());
+    dev_free(x);
+    dev_free(y);
+    return rc;
+}
+
 // Format statistics for the autotuner and reports: [0] padded SELL-C entries,
 // [1] problematic SELL-C rows, [2] segments, [3] (segment, row) pairs.
 extern "C" int gbk_matrix_format_stats(gbk_matrix A, int transposed, int64_t* stats) {
@@ -415,7 +553,7 @@ extern "C" int gbk_matrix_format_stats(gbk_matrix A, int transposed, int64_t* st
     for (int i = 0; i < 4; ++i) stats[i] = 0;
     if (A->sell[t]) {
         stats[0] = A->sell[t]->padded;
-        stats[1] = A->sell[t]->nproblem;
+        stats[1] = A->sell[t]->nproblem + A->sell[t]->nlong;
     }
     const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
     if (sg) {

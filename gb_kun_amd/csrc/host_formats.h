@@ -15,7 +15,8 @@ struct SellCHost {
     std::vector<int32_t> row_len;     // nchunks * C: row length (0 for rows of problematic chunks)
     std::vector<int32_t> col;         // padded, -1 = padding
     std::vector<char> val;            // padded * vsize
-    std::vector<int32_t> problem_rows;
+    std::vector<int32_t> problem_rows;  // rows of problematic chunks with at most long_thr entries
+    std::vector<int32_t> long_rows;     // rows with more than long_thr entries
 };
 
 struct SegHost {
@@ -32,7 +33,7 @@ struct SegHost {
 };
 
 void build_sell_c(int64_t n, const int32_t* rp, const int32_t* col, const void* val, size_t vs, int C, int sigma,
-                  SellCHost& out);
+                  int64_t long_thr, SellCHost& out);
 void build_segmented(int64_t n, const int32_t* rp, const int32_t* col, const void* val, size_t vs,
                      const std::vector<int64_t>& seg_bounds, const int32_t* col_relabel, SegHost& out);
 void lav_relabel(int64_t n, const int32_t* rp, const int32_t* col, double dense_frac, int64_t seg_cols,

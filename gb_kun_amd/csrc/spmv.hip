@@ -88,7 +88,8 @@ __global__ __launch_bounds__(MP_THREADS) void k_spmv_merge(const int32_t* __rest
                                                            const T* __restrict__ val, const T* __restrict__ x,
                                                            T* __restrict__ y, int32_t nrows,
                                                            const int2* __restrict__ tiles, T* __restrict__ carry,
-                                                           int accum, const int32_t* __restrict__ tile_map) {
+                                                           int accum, const int32_t* __restrict__ tile_map,
+                                                           const int32_t* __restrict__ row_map) {
     __shared__ int32_t s_end[MP_TILE + 1];
     __shared__ T s_prod[MP_TILE];
     __shared__ T s_y[MP_TILE];
@@ -203,9 +204,11 @@ __global__ __launch_bounds__(MP_THREADS) void k_spmv_merge(const int32_t* __rest
         carry[2 * tile] = tv;
         if (defer0) carry[2 * tile + 1] = s_y[0];
     }
+    // row_map names the output row of each CSR row (a CSR over a subset of
+    // the rows, e.g. the long rows of a SELL-C matrix)
     for (int i = tid; i < nr; i += MP_THREADS) {
         if (i == 0 && defer0) continue;
-        const int r = r0 + i;
+        const int r = row_map ? row_map[r0 + i] : r0 + i;
         const T res = s_y[i];
         y[r] = (accum == GBK_ACC_NONE) ? res : binop<T>(accum, y[r], res);
     }
@@ -213,7 +216,8 @@ __global__ __launch_bounds__(MP_THREADS) void k_spmv_merge(const int32_t* __rest
 
 template <typename T, int ADD>
 __global__ void k_spmv_merge_fixup(const int32_t* __restrict__ rp, const int2* __restrict__ tiles, int64_t ntiles,
-                                   const T* __restrict__ carry, T* __restrict__ y, int accum) {
+                                   const T* __restrict__ carry, T* __restrict__ y, int accum,
+                                   const int32_t* __restrict__ row_map) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     const int2 c0 = tiles[t], c1 = tiles[t + 1];
@@ -228,7 +232,8 @@ __global__ void k_spmv_merge_fixup(const int32_t* __restrict__ rp, const int2* _
         if (tiles[u].y <= start) break;
         --u;
     }
-    y[r0] = (accum == GBK_ACC_NONE) ? total : binop<T>(accum, y[r0], total);
+    const int ro = row_map ? row_map[r0] : r0;
+    y[ro] = (accum == GBK_ACC_NONE) ? total : binop<T>(accum, y[ro], total);
 }
 
 // Row-group kernel: G lanes per row.  rows == nullptr: all rows (count = n);
@@ -301,15 +306,15 @@ static int launch_rows(const DevCSR& m, const T* x, T* y, int64_t count, const i
 
 template <typename T, int ADD, int MUL>
 static int launch_merge(const DevCSR& m, const T* x, T* y, int accum, const int32_t* tile_map = nullptr,
-                        void* carry_ws = nullptr) {
+                        void* carry_ws = nullptr, const int32_t* row_map = nullptr) {
     if (m.ntiles == 0) return GBK_SUCCESS;
     hipStream_t st = stream();
     T* carry = (T*)(carry_ws ? carry_ws : m.carry);
     hipLaunchKernelGGL((k_spmv_merge<T, ADD, MUL>), dim3((unsigned)m.ntiles), dim3(MP_THREADS), 0, st, m.row_ptr, m.col,
-                       (const T*)m.val, x, y, (int32_t)m.nrows, m.tiles, carry, accum, tile_map);
+                       (const T*)m.val, x, y, (int32_t)m.nrows, m.tiles, carry, accum, tile_map, row_map);
     GBK_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_spmv_merge_fixup<T, ADD>), dim3((unsigned)((m.ntiles + 255) / 256)), dim3(256), 0, st,
-                       m.row_ptr, m.tiles, m.ntiles, (const T*)carry, y, accum);
+                       m.row_ptr, m.tiles, m.ntiles, (const T*)carry, y, accum, row_map);
     GBK_LAUNCH_CHECK();
     return GBK_SUCCESS;
 }
@@ -344,28 +349,30 @@ int spmv_merge_ws(const DevCSR& m, int type, const void* x, void* y, int add, in
 
 // Merge-path SpMV of a CSR with an optional tile order (used by the segmented formats).
 int spmv_merge_mapped(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
-                      const int32_t* tile_map) {
+                      const int32_t* tile_map, const int32_t* row_map) {
     return dispatch_type(type, [&](auto tv) -> int {
         typedef decltype(tv) T;
         return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
             constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
             if (order_sensitive(ADD))
                 return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, m.nrows, nullptr, nullptr, 0, 1, accum, 1);
-            return launch_merge<T, ADD, MUL>(m, (const T*)x, (T*)y, accum, tile_map);
+            if (row_map && order_sensitive(ADD))
+                return fail(GBK_INVALID_VALUE, "order-sensitive monoid on a row-mapped CSR");
+            return launch_merge<T, ADD, MUL>(m, (const T*)x, (T*)y, accum, tile_map, nullptr, row_map);
         });
     });
 }
 
-// Row-group SpMV over a list of rows of a CSR (G = 64 lanes per row unless the
-// monoid is order-sensitive).
+// Row-group SpMV over a list of rows of a CSR: `group` lanes per row (64 when
+// group is 0; always 1, the sequential fold, for order-sensitive monoids).
 int spmv_rows_list(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum,
-                   const int32_t* rows, int64_t count) {
+                   const int32_t* rows, int64_t count, int group) {
     return dispatch_type(type, [&](auto tv) -> int {
         typedef decltype(tv) T;
         return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
             constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
-            return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, count, rows, nullptr, 0, 1, accum,
-                                            order_sensitive(ADD) ? 1 : 64);
+            const int G = order_sensitive(ADD) ? 1 : (group > 0 ? group : 64);
+            return launch_rows<T, ADD, MUL>(m, (const T*)x, (T*)y, count, rows, nullptr, 0, 1, accum, G);
         });
     });
 }

@@ -213,6 +213,12 @@ int gbk_matrix_device_csr(gbk_matrix A, int transposed, const int32_t** row_ptr,
                           const int32_t** col_ids, const void** vals);
 /* Bytes of HBM held by the matrix (all formats and workspaces). */
 int gbk_matrix_device_bytes(gbk_matrix A, int64_t* bytes);
+/* Format autotuner (no reference counterpart: the reference takes the format
+ * from the command line, cmd_parser.hpp:21-112): builds CSR, segmented CSR,
+ * LAV and SELL-C-sigma (C = 64 and 32, sigma = 512) in turn, times `trials`
+ * SpMVs of each on the device, and keeps the fastest as the matrix's format.
+ * times_us (may be NULL) receives the five average times in that order. */
+int gbk_matrix_autotune(gbk_matrix A, int trials, int* chosen_format, int* chosen_c, double* times_us);
 /* Statistics of the optimized representation of one orientation (the
  * reference's MatrixSellC::print_stats / segment counts): stats[0] padded
  * SELL-C entries, [1] rows of problematic SELL-C chunks, [2] segments,

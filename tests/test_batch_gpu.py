@@ -25,14 +25,9 @@ def _sources(rp, k, seed):
     return rng.choice(cand, size=min(k, len(cand)), replace=False)
 
 
-def _torch():
-    import torch
-    return torch
-
-
 @pytest.mark.parametrize("k", [1, 3, 8, 17, 32, 64])
 def test_spmm_int_exact(gbk, oracle, k):
-    torch = _torch()
+    from gb_kun_amd.device import DeviceArray
     graphs = {"rmat13": gbk.gen_rmat(13, 16, seed=81), "er12": gbk.gen_uniform(12, 8, seed=82)}
     graphs.update({k2: v for k2, v in special_graphs().items() if k2 in ("hub_row", "tile_edges", "empty_run")})
     for gname, (rp, col) in graphs.items():
@@ -42,12 +37,10 @@ def test_spmm_int_exact(gbk, oracle, k):
         X = (gbk.uniform_f32(84, n * k).reshape(n, k) * 1000).astype(np.int32)
         for sr in ("PlusMultiplies", "MinimumPlus"):
             add, mul = gbk.SEMIRINGS[sr]
-            Xd = torch.from_numpy(X).cuda()
-            Yd = torch.zeros((n, k), dtype=torch.int32, device="cuda")
-            torch.cuda.synchronize()
-            gbk.spmm_device(A, Xd.data_ptr(), Yd.data_ptr(), k, sr)
-            gbk.synchronize()
-            Y = Yd.cpu().numpy()
+            Xd = DeviceArray.from_numpy(X)
+            Yd = DeviceArray.zeros((n, k), np.int32)
+            gbk.spmm_device(A, Xd.ptr, Yd.ptr, k, sr)
+            Y = Yd.numpy()
             for q in sorted({0, k // 2, k - 1}):
                 want = oracle.spmv(rp, col, vals, X[:, q].copy(), np.zeros(n, np.int32), add, mul)
                 assert np.array_equal(Y[:, q], want), f"{gname} {sr} k={k} column {q}"
@@ -55,19 +48,17 @@ def test_spmm_int_exact(gbk, oracle, k):
 
 @pytest.mark.parametrize("k", [2, 32])
 def test_spmm_fp32_and_accum(gbk, oracle, k):
-    torch = _torch()
+    from gb_kun_amd.device import DeviceArray
     rp, col = gbk.gen_rmat(14, 16, seed=85)
     n = len(rp) - 1
     vals = gbk.uniform_f32(86, len(col))
     A = gbk.Matrix.from_csr(rp, col, vals)
     X = gbk.uniform_f32(87, n * k).reshape(n, k)
     Y0 = gbk.uniform_f32(88, n * k).reshape(n, k)
-    Xd = torch.from_numpy(X).cuda()
-    Yd = torch.from_numpy(Y0.copy()).cuda()
-    torch.cuda.synchronize()
-    gbk.spmm_device(A, Xd.data_ptr(), Yd.data_ptr(), k, "PlusMultiplies", accum=gbk.ACC_PLUS)
-    gbk.synchronize()
-    Y = Yd.cpu().numpy()
+    Xd = DeviceArray.from_numpy(X)
+    Yd = DeviceArray.from_numpy(Y0)
+    gbk.spmm_device(A, Xd.ptr, Yd.ptr, k, "PlusMultiplies", accum=gbk.ACC_PLUS)
+    Y = Yd.numpy()
     for q in range(k):
         y64, absum = oracle.spmv_f64(rp, col, vals, X[:, q].copy())
         err = np.abs(Y[:, q].astype(np.float64) - (Y0[:, q].astype(np.float64) + y64))

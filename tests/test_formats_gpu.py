@@ -2,9 +2,10 @@
 
 Every format must give the CSR result: bit-exact for integer semirings; for
 fp32 plus.times every format is within 1e-5 relative of the FP64 oracle.
-SELL-C folds the rows of its regular chunks in CSR order, so on a graph
-without problematic chunks it reproduces the reference's CSR bits (checked
-on the golden vector of the compiled reference).
+SELL-C folds every row of at most 4C entries sequentially in CSR order (its
+chunk rows and the rows of problematic chunks), so on a graph without longer
+rows it reproduces the reference's CSR bits (checked on the golden vector of
+the compiled reference, whose longest row has 212 entries).
 """
 import numpy as np
 import pytest

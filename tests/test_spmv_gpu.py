@@ -147,18 +147,18 @@ def test_storage_after_mxv(gbk, oracle):
 
 def test_spmv_device_repeat_deterministic(gbk, oracle):
     """The raw-pointer seam: repeated calls give bit-identical fp32 results."""
-    import torch
+    from gb_kun_amd.device import DeviceArray
     rp, col = gbk.gen_rmat(13, 16, seed=8)
     vals = gbk.uniform_f32(61, len(col))
     A = gbk.Matrix.from_csr(rp, col, vals)
-    x = torch.from_numpy(gbk.uniform_f32(62, A.n)).cuda()
-    y = torch.zeros(A.n, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()  # the library works on its own stream
+    xh = gbk.uniform_f32(62, A.n)
+    x = DeviceArray.from_numpy(xh)
+    y = DeviceArray.zeros(A.n, np.float32)
     outs = []
     for _ in range(3):
-        gbk.spmv_device(A, x.data_ptr(), y.data_ptr())
+        gbk.spmv_device(A, x.ptr, y.ptr)
         gbk.synchronize()
-        outs.append(y.cpu().numpy().copy())
+        outs.append(y.numpy())
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
-    want, absum = oracle.spmv_f64(rp, col, vals, x.cpu().numpy())
+    want, absum = oracle.spmv_f64(rp, col, vals, xh)
     assert np.all(np.abs(outs[0] - want) <= REL_TOL * absum + 1e-30)
