@@ -137,6 +137,14 @@ _sig("gbk_msbfs_device", C.c_int, _vp, _pi64, C.c_int, _vp, _pi64)
 _sig("gbk_ppr_multi", C.c_int, _vp, _pi64, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double,
      C.POINTER(C.c_int))
 _sig("gbk_sssp_multi", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
+_sig("gbk_server_new", C.c_int, C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int)
+_sig("gbk_server_free", C.c_int, _vp)
+_sig("gbk_server_add_graph", C.c_int, _vp, _vp, _vp, _vp, C.POINTER(C.c_int))
+_sig("gbk_server_submit", C.c_int, _vp, C.c_int, C.c_int, C.c_int64, C.c_double, _pi64)
+_sig("gbk_server_run", C.c_int, _vp)
+_sig("gbk_server_query_info", C.c_int, _vp, C.c_int64, C.POINTER(C.c_double), _pi64)
+_sig("gbk_server_result", C.c_int, _vp, C.c_int64, _vp)
+_sig("gbk_server_stats", C.c_int, _vp, C.POINTER(C.c_double))
 _sig("gbk_gen_rmat", C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
      C.c_int, _pi64, _pi64, C.POINTER(_pi64), C.POINTER(_pi32))
 _sig("gbk_gen_uniform", C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _pi64, _pi64, C.POINTER(_pi64),
@@ -483,6 +491,65 @@ def sssp_multi(A, sources):
     st = np.zeros(8, dtype=np.int64)
     check(lib.gbk_sssp_multi(A.h, _ptr(src, C.c_int64), len(src), _ptr(d, C.c_int32), _ptr(st, C.c_int64)))
     return d, st[:2]
+
+
+# ------------------------------------------------------------ query server
+QUERY_BFS, QUERY_PPR, QUERY_SSSP = 0, 1, 2
+
+
+class Server:
+    """Continuous-batching query server over device-resident graphs (gbk_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, ppr_slots=32, sssp_slots=32, damping=0.85, tol=1e-6, itermax=100,
+                 keep_results=False):
+        self.h = _vp()
+        check(lib.gbk_server_new(C.byref(self.h), int(hbm_budget_bytes), ppr_slots, sssp_slots, damping, tol,
+                                 itermax, int(keep_results)))
+        self.graph_n = []
+
+    def add_graph(self, bfs=None, ppr=None, sssp=None):
+        gid = C.c_int(0)
+        check(lib.gbk_server_add_graph(self.h, bfs.h if bfs else None, ppr.h if ppr else None,
+                                       sssp.h if sssp else None, C.byref(gid)))
+        self.graph_n.append(next(m for m in (bfs, ppr, sssp) if m is not None).n)
+        return gid.value
+
+    def submit(self, graph, kind, source, arrival=0.0):
+        q = C.c_int64(0)
+        check(lib.gbk_server_submit(self.h, graph, kind, int(source), float(arrival), C.byref(q)))
+        return q.value
+
+    def run(self):
+        check(lib.gbk_server_run(self.h))
+
+    def info(self, qid):
+        t = (C.c_double * 4)()
+        i = (C.c_int64 * 4)()
+        check(lib.gbk_server_query_info(self.h, qid, t, i))
+        return {"arrival": t[0], "admit": t[1], "done": t[2], "checksum": t[3], "reached": i[0],
+                "occupancy": i[1], "slot": i[2], "status": i[3]}
+
+    def result(self, qid, graph, kind):
+        out = np.empty(self.graph_n[graph], dtype=np.float32 if kind == QUERY_PPR else np.int32)
+        check(lib.gbk_server_result(self.h, qid, _ptr(out)))
+        return out
+
+    def stats(self):
+        st = (C.c_double * 5)()
+        check(lib.gbk_server_stats(self.h, st))
+        return {"wall_s": st[0], "queries": int(st[1]), "peak_bytes": int(st[2]), "engines": int(st[3]),
+                "max_engines_live": int(st[4])}
+
+    def free(self):
+        if self.h:
+            lib.gbk_server_free(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------- element-wise ops

@@ -2,6 +2,9 @@
 // server built on them (serve.cpp).  Not part of the C ABI.
 #pragma once
 
+#include <memory>
+#include <vector>
+
 #include "gbk_objects.h"
 
 namespace gbk {
@@ -37,5 +40,26 @@ int spmv_merge_ws(const DevCSR& m, int type, const void* x, void* y, int add, in
 size_t msbfs_ws_bytes(int64_t n);
 int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t ws_bytes, int32_t* levels_out,
               int64_t* stats, hipStream_t st);
+
+// One (graph, query kind) batch of the query server: up to slots() queries in
+// flight, advanced together one level / iteration / round per step() on the
+// engine's stream.  A query admitted into a free slot joins at the next step;
+// step() reports the slots whose query completed in it; extract() writes a
+// slot's result (result_bytes()) to device memory; release() frees the slot.
+struct Engine {
+    virtual ~Engine() {}
+    virtual int slots() const = 0;
+    virtual size_t bytes() const = 0;
+    virtual size_t result_bytes() const = 0;
+    virtual int admit(int slot, int32_t source) = 0;
+    virtual int step(std::vector<int>& finished) = 0;
+    virtual int extract(int slot, void* out_dev) = 0;
+    virtual void release(int slot) = 0;
+};
+
+int make_bfs_engine(gbk_matrix A, hipStream_t st, std::unique_ptr<Engine>* out);
+int make_ppr_engine(gbk_matrix A, hipStream_t st, int k, double damping, double tol, int itermax,
+                    std::unique_ptr<Engine>* out);
+int make_sssp_engine(gbk_matrix A, hipStream_t st, int k, std::unique_ptr<Engine>* out);
 
 }  // namespace gbk

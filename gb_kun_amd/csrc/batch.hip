@@ -26,6 +26,7 @@
 //          BFS.  Levels are 1-based with 0 = unreached, exactly as the
 //          single-source driver (algorithms/bfs/bfs.hpp:94-142) reports them.
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "gbk_objects.h"
@@ -245,7 +246,8 @@ __global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ 
                                                     int64_t n, uint64_t all, const uint64_t* __restrict__ F,
                                                     uint64_t* __restrict__ V, uint64_t* __restrict__ N,
                                                     int32_t* __restrict__ lev, int32_t next_level,
-                                                    unsigned long long* __restrict__ counters) {
+                                                    unsigned long long* __restrict__ counters,
+                                                    unsigned long long* __restrict__ active) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
@@ -280,9 +282,15 @@ __global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ 
         }
     }
     unsigned long long c = (v < n && lane == 0 && found) ? (unsigned long long)__popcll(found) : 0ull;
+    unsigned long long m = (v < n && lane == 0) ? (unsigned long long)found : 0ull;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off, 64);
+        m |= __shfl_xor(m, off, 64);
+    }
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(counters, c);
+    // searches that found a vertex in this step
+    if (active && (threadIdx.x & 63) == 0 && m) atomicOr(active, m);
 }
 
 // Compacts the vertices with a non-empty frontier word into a queue and sums
@@ -339,9 +347,10 @@ __global__ __launch_bounds__(256) void k_msbfs_push(const int32_t* __restrict__ 
 // After a push step: new = N & ~V per vertex; marks visited and levels.
 __global__ __launch_bounds__(256) void k_msbfs_settle(int64_t n, uint64_t* __restrict__ V, uint64_t* __restrict__ N,
                                                       int32_t* __restrict__ lev, int32_t next_level,
-                                                      unsigned long long* __restrict__ counters) {
+                                                      unsigned long long* __restrict__ counters,
+                                                      unsigned long long* __restrict__ active) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long c = 0;
+    unsigned long long c = 0, m = 0;
     if (v < n) {
         const uint64_t vis = V[v];
         const uint64_t nw = N[v] & ~vis;
@@ -355,11 +364,16 @@ __global__ __launch_bounds__(256) void k_msbfs_settle(int64_t n, uint64_t* __res
                 b &= b - 1;
             }
             c = (unsigned long long)__popcll(nw);
+            m = nw;
         }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off, 64);
+        m |= __shfl_xor(m, off, 64);
+    }
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(counters, c);
+    if (active && (threadIdx.x & 63) == 0 && m) atomicOr(active, m);
 }
 
 __global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
@@ -486,14 +500,15 @@ int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t w
                                    (int64_t)ne, F, V, N);
                 GBK_LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_msbfs_settle, grid_of(n), dim3(256), 0, st, n, V, N, lev, cur + 1, ctr);
+            hipLaunchKernelGGL(k_msbfs_settle, grid_of(n), dim3(256), 0, st, n, V, N, lev, cur + 1, ctr,
+                               (unsigned long long*)nullptr);
             GBK_LAUNCH_CHECK();
         } else {
             ++pulls;
 #define GBK_MSPULL(GG)                                                                                         \
     case GG:                                                                                                   \
         hipLaunchKernelGGL(k_msbfs_pull<GG>, grid_of(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, all, F, V, \
-                           N, lev, cur + 1, ctr);                                                              \
+                           N, lev, cur + 1, ctr, (unsigned long long*)nullptr);                                \
         break;
             switch (G) {
                 GBK_MSPULL(4)
@@ -522,6 +537,206 @@ int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t w
         stats[3] = edges_scanned;
         stats[4] = reached;
     }
+    return GBK_SUCCESS;
+}
+
+
+// ------------------------------------------------------ BFS serving engine --
+// Multi-source BFS as a continuously refilled batch: 64 slots, one search
+// per slot.  A slot that finishes (its frontier is empty) is retired and can
+// take a new search at the next step; levels are stored as global step
+// numbers and rebased per slot on extraction.
+
+// Clears the search bits of `mask` in every vertex (a slot being reused).
+__global__ void k_bfs_slot_reset(int64_t n, uint64_t mask, uint64_t* __restrict__ F, uint64_t* __restrict__ V,
+                                 uint64_t* __restrict__ N, int32_t* __restrict__ lev) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    F[v] &= ~mask;
+    N[v] &= ~mask;
+    const uint64_t had = V[v] & mask;
+    if (had) {
+        V[v] &= ~mask;
+        uint64_t b = had;
+        while (b) {
+            const int q = __ffsll((long long)b) - 1;
+            lev[v * 64 + q] = 0;
+            b &= b - 1;
+        }
+    }
+}
+
+// Seeds search `slot[i]` at vertex `src[i]` (one thread per seed).
+__global__ void k_bfs_seed(const int32_t* __restrict__ src, const int32_t* __restrict__ slot, int count,
+                           uint64_t* __restrict__ F, uint64_t* __restrict__ V, int32_t* __restrict__ lev, int32_t cur) {
+    const int i = threadIdx.x;
+    if (i < count) {
+        const int64_t s = src[i];
+        const int q = slot[i];
+        atomicOr((unsigned long long*)(F + s), 1ull << q);
+        atomicOr((unsigned long long*)(V + s), 1ull << q);
+        lev[s * 64 + q] = cur;
+    }
+}
+
+// out[v] = level of v in search q (1-based, 0 = unreached).
+__global__ void k_bfs_extract(const int32_t* __restrict__ lev, int64_t n, int q, int32_t base, int32_t* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) {
+        const int32_t l = lev[v * 64 + q];
+        out[v] = l ? l - base + 1 : 0;
+    }
+}
+
+namespace {
+
+class BfsEngine : public Engine {
+  public:
+    BfsEngine(gbk_matrix A, hipStream_t st) : A_(A), st_(st), n_(A->n) {}
+    ~BfsEngine() override {
+        dev_free(ws_);
+        dev_free(seeds_);
+        if (hctr_) (void)hipHostFree(hctr_);
+    }
+    int init() {
+        ws_bytes_ = msbfs_ws_bytes(n_) + 512;
+        GBK_TRY(dev_alloc(&ws_, ws_bytes_));
+        char* base = (char*)(((uintptr_t)ws_ + 255) & ~(uintptr_t)255);
+        L_ = msbfs_layout(n_, base);
+        GBK_TRY(dev_alloc((void**)&seeds_, sizeof(int32_t) * 128));
+        GBK_HIP(hipHostMalloc((void**)&hctr_, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+        hipLaunchKernelGGL(k_msbfs_init, grid_of(n_), dim3(256), 0, st_, L_.F, L_.V, n_);
+        GBK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_msbfs_init, grid_of(n_), dim3(256), 0, st_, L_.N, L_.N, n_);
+        GBK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_zero_u64, grid_of(32 * n_), dim3(256), 0, st_, (uint64_t*)L_.lev, 32 * n_);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipStreamSynchronize(st_));
+        int G = 4;
+        const double avg = n_ ? (double)A_->csc.nnz / (double)n_ : 0;
+        while (G < 32 && G < avg) G <<= 1;
+        G_ = G;
+        return GBK_SUCCESS;
+    }
+    int slots() const override { return 64; }
+    size_t bytes() const override { return ws_bytes_; }
+    size_t result_bytes() const override { return sizeof(int32_t) * (size_t)n_; }
+    int admit(int slot, int32_t source) override {
+        if (source < 0 || source >= n_) return fail(GBK_INVALID_INDEX, "source out of range");
+        pend_src_.push_back(source);
+        pend_slot_.push_back(slot);
+        return GBK_SUCCESS;
+    }
+    int step(std::vector<int>& finished) override {
+        finished.clear();
+        const DevCSR& out = A_->csr;
+        const DevCSR& in = A_->csc;
+        unsigned long long* ctr = L_.ctr;
+        if (!pend_slot_.empty()) {
+            uint64_t mask = 0;
+            for (int q : pend_slot_) mask |= 1ull << q;
+            hipLaunchKernelGGL(k_bfs_slot_reset, grid_of(n_), dim3(256), 0, st_, n_, mask, L_.F, L_.V, L_.N, L_.lev);
+            GBK_LAUNCH_CHECK();
+            std::vector<int32_t> buf(pend_src_);
+            buf.insert(buf.end(), pend_slot_.begin(), pend_slot_.end());
+            GBK_HIP(hipMemcpyAsync(seeds_, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, st_));
+            const int cnt = (int)pend_slot_.size();
+            hipLaunchKernelGGL(k_bfs_seed, dim3(1), dim3(64), 0, st_, seeds_, seeds_ + cnt, cnt, L_.F, L_.V, L_.lev, cur_);
+            GBK_LAUNCH_CHECK();
+            for (int q : pend_slot_) {
+                base_[q] = cur_;
+                occupied_ |= 1ull << q;
+            }
+            GBK_HIP(hipStreamSynchronize(st_));  // buf leaves scope
+            pend_src_.clear();
+            pend_slot_.clear();
+        }
+        if (!occupied_) return GBK_SUCCESS;
+        // frontier queue and its out-edges decide the direction
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st_, (uint64_t*)ctr, (int64_t)4);
+        GBK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_msbfs_queue, grid_of(n_), dim3(256), 0, st_, out.row_ptr, L_.F, n_, L_.q,
+                           (int32_t*)(ctr + 2), ctr + 1);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipMemcpyAsync(hctr_, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
+        GBK_HIP(hipStreamSynchronize(st_));
+        const int64_t nq = (int64_t)(hctr_[2] & 0xffffffffull);
+        const bool push = (double)hctr_[1] * 14.0 < (double)out.nnz;
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st_, (uint64_t*)ctr, (int64_t)4);
+        GBK_LAUNCH_CHECK();
+        if (nq > 0) {
+            if (push) {
+                hipLaunchKernelGGL(k_zero_u64, grid_of(n_), dim3(256), 0, st_, L_.N, n_);
+                GBK_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_queue_degrees, grid_of(nq), dim3(256), 0, st_, out.row_ptr, L_.q, nq, L_.deg);
+                GBK_LAUNCH_CHECK();
+                int32_t ne = 0;
+                GBK_TRY(exclusive_scan_i32_on(L_.deg, L_.offs, nq, &ne, st_, L_.partial, L_.part_bytes));
+                if (ne > 0) {
+                    hipLaunchKernelGGL(k_msbfs_push, grid_of(ne), dim3(256), 0, st_, out.row_ptr, out.col, L_.q, nq,
+                                       L_.offs, (int64_t)ne, L_.F, L_.V, L_.N);
+                    GBK_LAUNCH_CHECK();
+                }
+                hipLaunchKernelGGL(k_msbfs_settle, grid_of(n_), dim3(256), 0, st_, n_, L_.V, L_.N, L_.lev, cur_ + 1,
+                                   ctr, ctr + 3);
+                GBK_LAUNCH_CHECK();
+            } else {
+#define GBK_SPULL(GG)                                                                                       \
+    case GG:                                                                                                \
+        hipLaunchKernelGGL(k_msbfs_pull<GG>, grid_of(n_ * GG), dim3(256), 0, st_, in.row_ptr, in.col, n_,    \
+                           (uint64_t)occupied_, L_.F, L_.V, L_.N, L_.lev, cur_ + 1, ctr, ctr + 3);          \
+        break;
+                switch (G_) {
+                    GBK_SPULL(4)
+                    GBK_SPULL(8)
+                    GBK_SPULL(16)
+                    GBK_SPULL(32)
+                }
+#undef GBK_SPULL
+                GBK_LAUNCH_CHECK();
+            }
+            GBK_HIP(hipMemcpyAsync(hctr_, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
+            GBK_HIP(hipStreamSynchronize(st_));
+        } else {
+            hctr_[3] = 0;
+        }
+        const uint64_t active = hctr_[3] & occupied_;
+        for (int q = 0; q < 64; ++q)
+            if (((occupied_ >> q) & 1ull) && !((active >> q) & 1ull)) finished.push_back(q);
+        std::swap(L_.F, L_.N);
+        ++cur_;
+        return GBK_SUCCESS;
+    }
+    int extract(int slot, void* out_dev) override {
+        hipLaunchKernelGGL(k_bfs_extract, grid_of(n_), dim3(256), 0, st_, L_.lev, n_, slot, base_[slot],
+                           (int32_t*)out_dev);
+        GBK_LAUNCH_CHECK();
+        return GBK_SUCCESS;
+    }
+    void release(int slot) override { occupied_ &= ~(1ull << slot); }
+
+  private:
+    gbk_matrix A_;
+    hipStream_t st_;
+    int64_t n_;
+    void* ws_ = nullptr;
+    int32_t* seeds_ = nullptr;  // source, slot pairs of the next seeding (<= 64 each)
+    size_t ws_bytes_ = 0;
+    MsbfsLayout L_{};
+    unsigned long long* hctr_ = nullptr;
+    uint64_t occupied_ = 0;
+    int32_t cur_ = 1;
+    int32_t base_[64] = {0};
+    int G_ = 4;
+    std::vector<int32_t> pend_src_, pend_slot_;
+};
+
+}  // namespace
+
+int make_bfs_engine(gbk_matrix A, hipStream_t st, std::unique_ptr<Engine>* out) {
+    auto e = std::make_unique<BfsEngine>(A, st);
+    GBK_TRY(e->init());
+    *out = std::move(e);
     return GBK_SUCCESS;
 }
 

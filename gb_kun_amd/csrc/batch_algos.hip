@@ -18,6 +18,7 @@
 // edge for all k sources, int32 distances saturate at INT32_MAX.
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <vector>
 
 #include "batch.h"
@@ -384,12 +385,13 @@ static MsspLayout mssp_layout(int64_t n, int k, char* base) {
         return r;
     };
     L.D = (int32_t*)take(sizeof(int32_t) * (size_t)n * k);
+    // queues hold at most n distinct vertices plus up to 64 seeds
     L.inq = (int32_t*)take(sizeof(int32_t) * (n + 1));
-    L.q0 = (int32_t*)take(sizeof(int32_t) * (n + 1));
-    L.q1 = (int32_t*)take(sizeof(int32_t) * (n + 1));
-    L.deg = (int32_t*)take(sizeof(int32_t) * (n + 1));
-    L.offs = (int32_t*)take(sizeof(int32_t) * (n + 1));
-    L.part_bytes = scan_partial_bytes(n);
+    L.q0 = (int32_t*)take(sizeof(int32_t) * (n + 66));
+    L.q1 = (int32_t*)take(sizeof(int32_t) * (n + 66));
+    L.deg = (int32_t*)take(sizeof(int32_t) * (n + 66));
+    L.offs = (int32_t*)take(sizeof(int32_t) * (n + 66));
+    L.part_bytes = scan_partial_bytes(n + 66);
     L.partial = take(L.part_bytes);
     L.cnt = (int32_t*)take(sizeof(int32_t) * 4);
     L.src = (int32_t*)take(sizeof(int32_t) * 64);
@@ -461,6 +463,307 @@ int mssp_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_b
         stats[0] = rounds;
         stats[1] = relaxed;
     }
+    return GBK_SUCCESS;
+}
+
+
+// --------------------------------------------------- PPR / SSSP engines --
+// Column q of the dense block belongs to slot q.  Seeding a slot rewrites its
+// column; the other columns are untouched, so a query computes the same
+// numbers alone and in a mix (the SpMM folds every column in the same order
+// whatever the other columns hold).
+
+// R[v][slot[i]] = (v == src[i]) for every seeded slot i.
+__global__ void k_ppr_seed(float* __restrict__ R, int64_t n, int k, const int32_t* __restrict__ src,
+                           const int32_t* __restrict__ slot, int count) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    for (int i = 0; i < count; ++i) R[v * k + slot[i]] = (src[i] == v) ? 1.0f : 0.0f;
+}
+
+template <typename T>
+__global__ void k_extract_col(const T* __restrict__ M, int64_t n, int k, int q, T* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) out[v] = M[v * k + q];
+}
+
+// D[v][slot[i]] = (v == src[i]) ? 0 : INT32_MAX; seeds join the queue at qa[nq + i].
+__global__ void k_sssp_seed(int32_t* __restrict__ D, int64_t n, int k, const int32_t* __restrict__ src,
+                            const int32_t* __restrict__ slot, int count, int32_t* __restrict__ qa, int64_t nq) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n)
+        for (int i = 0; i < count; ++i) D[v * k + slot[i]] = (src[i] == v) ? 0 : INT32_MAX;
+    if (v < count) qa[nq + v] = src[v];
+}
+
+namespace {
+
+class PprEngine : public Engine {
+  public:
+    PprEngine(gbk_matrix A, hipStream_t st, int k, double damping, double tol, int itermax)
+        : A_(A), st_(st), n_(A->n), k_(k), damping_(damping), tol_(tol), itermax_(itermax) {}
+    ~PprEngine() override {
+        dev_free(ws_);
+        dev_free(seeds_);
+        if (hdelta_) (void)hipHostFree(hdelta_);
+    }
+    int init() {
+        if (A_->type != GBK_FP32) return fail(GBK_DOMAIN_MISMATCH, "personalized PageRank needs an FP32 matrix");
+        GBK_TRY(ensure_spmm_tiles(A_->csr, st_));
+        ws_bytes_ = ppr_ws_bytes(A_, k_) + 512;
+        GBK_TRY(dev_alloc(&ws_, ws_bytes_));
+        GBK_TRY(dev_alloc((void**)&seeds_, sizeof(int32_t) * 128));
+        GBK_HIP(hipHostMalloc((void**)&hdelta_, sizeof(double) * 64, hipHostMallocDefault));
+        char* base = (char*)(((uintptr_t)ws_ + 255) & ~(uintptr_t)255);
+        L_ = ppr_layout(A_->csr, n_, k_, base);
+        hipLaunchKernelGGL(k_sink_flags, grid_b(n_), dim3(256), 0, st_, A_->csc.row_ptr, n_, L_.flag);
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(exclusive_scan_i32_on(L_.flag, L_.offs, n_, &nsinks_, st_, L_.partial, L_.part_bytes));
+        hipLaunchKernelGGL(k_compact_flags, grid_b(n_), dim3(256), 0, st_, L_.flag, L_.offs, n_, L_.sinks);
+        GBK_LAUNCH_CHECK();
+        // every column starts frozen at zero
+        std::vector<int32_t> ones(64, 1), zeros(64, 0);
+        GBK_HIP(hipMemcpyAsync(L_.done, ones.data(), sizeof(int32_t) * 64, hipMemcpyHostToDevice, st_));
+        GBK_HIP(hipMemcpyAsync(L_.src, zeros.data(), sizeof(int32_t) * 64, hipMemcpyHostToDevice, st_));
+        hipLaunchKernelGGL(k_ppr_init, grid_b(n_ * k_), dim3(256), 0, st_, L_.R, n_, k_, L_.src);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipStreamSynchronize(st_));
+        for (int q = 0; q < 64; ++q) done_[q] = 1;
+        return GBK_SUCCESS;
+    }
+    int slots() const override { return k_; }
+    size_t bytes() const override { return ws_bytes_; }
+    size_t result_bytes() const override { return sizeof(float) * (size_t)n_; }
+    int admit(int slot, int32_t source) override {
+        if (source < 0 || source >= n_) return fail(GBK_INVALID_INDEX, "source out of range");
+        pend_src_.push_back(source);
+        pend_slot_.push_back(slot);
+        return GBK_SUCCESS;
+    }
+    int step(std::vector<int>& finished) override {
+        finished.clear();
+        bool flags_dirty = false;
+        if (!pend_slot_.empty()) {
+            std::vector<int32_t> buf(pend_src_);
+            buf.insert(buf.end(), pend_slot_.begin(), pend_slot_.end());
+            GBK_HIP(hipMemcpyAsync(seeds_, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, st_));
+            const int cnt = (int)pend_slot_.size();
+            hipLaunchKernelGGL(k_ppr_seed, grid_b(n_), dim3(256), 0, st_, L_.R, n_, k_, seeds_, seeds_ + cnt, cnt);
+            GBK_LAUNCH_CHECK();
+            for (int i = 0; i < cnt; ++i) {
+                const int q = pend_slot_[i];
+                src_[q] = pend_src_[i];
+                done_[q] = 0;
+                iters_[q] = 0;
+                occupied_ |= 1ull << q;
+            }
+            flags_dirty = true;
+            GBK_HIP(hipStreamSynchronize(st_));
+            pend_src_.clear();
+            pend_slot_.clear();
+        }
+        if (!occupied_) return GBK_SUCCESS;
+        if (flags_dirty) {
+            GBK_HIP(hipMemcpyAsync(L_.done, done_, sizeof(int32_t) * k_, hipMemcpyHostToDevice, st_));
+            GBK_HIP(hipMemcpyAsync(L_.src, src_, sizeof(int32_t) * k_, hipMemcpyHostToDevice, st_));
+        }
+        hipStream_t prev = stream();
+        set_thread_stream(st_);
+        int rc;
+        if (k_ == 1) rc = spmv_merge_ws(A_->csr, GBK_FP32, L_.R, L_.Y, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L_.carry);
+        else rc = spmm_run(A_->csr, GBK_FP32, L_.R, L_.Y, k_, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L_.carry, st_);
+        set_thread_stream(prev);
+        GBK_TRY(rc);
+        const int per = 256 / k_;
+        const int64_t ub = std::max<int64_t>(std::min<int64_t>((n_ + per - 1) / per, (int64_t)device_cu_count() * 32), 1);
+        const int64_t sb = std::min<int64_t>(((int64_t)nsinks_ + per - 1) / per, (int64_t)device_cu_count() * 8);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink, 64);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.delta, 64);
+        if (nsinks_ > 0 && sb > 0)
+            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st_, L_.R, k_, L_.sinks, (int64_t)nsinks_,
+                               L_.sink);
+        hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)ub), dim3(256), 0, st_, L_.R, L_.Y, L_.Y, n_, k_, L_.src, L_.sink,
+                           L_.done, (float)damping_, L_.delta);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipMemcpyAsync(hdelta_, L_.delta, sizeof(double) * k_, hipMemcpyDeviceToHost, st_));
+        GBK_HIP(hipStreamSynchronize(st_));
+        std::swap(L_.R, L_.Y);
+        bool changed = false;
+        for (int q = 0; q < k_; ++q) {
+            if (!((occupied_ >> q) & 1ull) || done_[q]) continue;
+            ++iters_[q];
+            if ((tol_ > 0.0 && hdelta_[q] < tol_) || iters_[q] >= itermax_) {
+                done_[q] = 1;
+                changed = true;
+                finished.push_back(q);
+            }
+        }
+        if (changed) GBK_HIP(hipMemcpyAsync(L_.done, done_, sizeof(int32_t) * k_, hipMemcpyHostToDevice, st_));
+        return GBK_SUCCESS;
+    }
+    int extract(int slot, void* out_dev) override {
+        hipLaunchKernelGGL(k_extract_col<float>, grid_b(n_), dim3(256), 0, st_, L_.R, n_, k_, slot, (float*)out_dev);
+        GBK_LAUNCH_CHECK();
+        return GBK_SUCCESS;
+    }
+    void release(int slot) override { occupied_ &= ~(1ull << slot); }
+    int iterations(int slot) const { return iters_[slot]; }
+
+  private:
+    gbk_matrix A_;
+    hipStream_t st_;
+    int64_t n_;
+    int k_;
+    double damping_, tol_;
+    int itermax_;
+    void* ws_ = nullptr;
+    int32_t* seeds_ = nullptr;
+    size_t ws_bytes_ = 0;
+    PprLayout L_{};
+    int32_t nsinks_ = 0;
+    double* hdelta_ = nullptr;
+    uint64_t occupied_ = 0;
+    int32_t done_[64];
+    int32_t src_[64] = {0};
+    int iters_[64] = {0};
+    std::vector<int32_t> pend_src_, pend_slot_;
+};
+
+class SsspEngine : public Engine {
+  public:
+    SsspEngine(gbk_matrix A, hipStream_t st, int k) : A_(A), st_(st), n_(A->n), k_(k) {}
+    ~SsspEngine() override {
+        dev_free(ws_);
+        dev_free(seeds_);
+        if (hbuf_) (void)hipHostFree(hbuf_);
+    }
+    int init() {
+        if (A_->type != GBK_INT32) return fail(GBK_DOMAIN_MISMATCH, "SSSP needs an INT32 weight matrix");
+        ws_bytes_ = mssp_ws_bytes(n_, k_) + 512;
+        GBK_TRY(dev_alloc(&ws_, ws_bytes_));
+        GBK_TRY(dev_alloc((void**)&seeds_, sizeof(int32_t) * 128));
+        GBK_HIP(hipHostMalloc((void**)&hbuf_, 64, hipHostMallocDefault));
+        char* base = (char*)(((uintptr_t)ws_ + 255) & ~(uintptr_t)255);
+        L_ = mssp_layout(n_, k_, base);
+        hipLaunchKernelGGL(k_mssp_init, grid_b(std::max<int64_t>(n_ * k_, n_)), dim3(256), 0, st_, L_.D, n_, k_, L_.inq);
+        GBK_LAUNCH_CHECK();
+        GBK_HIP(hipStreamSynchronize(st_));
+        qa_ = L_.q0;
+        qb_ = L_.q1;
+        LW_ = k_ <= 8 ? 8 : k_ <= 16 ? 16 : k_ <= 32 ? 32 : 64;
+        return GBK_SUCCESS;
+    }
+    int slots() const override { return k_; }
+    size_t bytes() const override { return ws_bytes_; }
+    size_t result_bytes() const override { return sizeof(int32_t) * (size_t)n_; }
+    int admit(int slot, int32_t source) override {
+        if (source < 0 || source >= n_) return fail(GBK_INVALID_INDEX, "source out of range");
+        pend_src_.push_back(source);
+        pend_slot_.push_back(slot);
+        return GBK_SUCCESS;
+    }
+    int step(std::vector<int>& finished) override {
+        finished.clear();
+        const DevCSR& m = A_->csr;
+        uint64_t seeded = 0;
+        if (!pend_slot_.empty()) {
+            std::vector<int32_t> buf(pend_src_);
+            buf.insert(buf.end(), pend_slot_.begin(), pend_slot_.end());
+            GBK_HIP(hipMemcpyAsync(seeds_, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, st_));
+            const int cnt = (int)pend_slot_.size();
+            hipLaunchKernelGGL(k_sssp_seed, grid_b(std::max<int64_t>(n_, cnt)), dim3(256), 0, st_, L_.D, n_, k_, seeds_,
+                               seeds_ + cnt, cnt, qa_, nq_);
+            GBK_LAUNCH_CHECK();
+            nq_ += cnt;
+            for (int q : pend_slot_) {
+                occupied_ |= 1ull << q;
+                seeded |= 1ull << q;
+            }
+            GBK_HIP(hipStreamSynchronize(st_));
+            pend_src_.clear();
+            pend_slot_.clear();
+        }
+        if (!occupied_) return GBK_SUCCESS;
+        uint64_t changed = 0;
+        if (nq_ > 0) {
+            hipLaunchKernelGGL(k_queue_deg, grid_b(nq_), dim3(256), 0, st_, m.row_ptr, qa_, nq_, L_.deg);
+            GBK_LAUNCH_CHECK();
+            int32_t ne = 0;
+            GBK_TRY(exclusive_scan_i32_on(L_.deg, L_.offs, nq_, &ne, st_, L_.partial, L_.part_bytes));
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st_, L_.cnt, 4);
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st_, (int32_t*)L_.changed, 4);
+            GBK_LAUNCH_CHECK();
+            if (ne > 0) {
+                const int64_t groups = ((int64_t)ne + 3) / 4;
+#define GBK_SMSSP(LWV)                                                                                        \
+    case LWV:                                                                                                 \
+        hipLaunchKernelGGL(k_mssp_relax<LWV>, grid_b(groups, 256 / LWV), dim3(256), 0, st_, m.row_ptr, m.col,  \
+                           (const int32_t*)m.val, qa_, nq_, L_.offs, (int64_t)ne, k_, L_.D, L_.inq, qb_, L_.cnt,  \
+                           round_, L_.changed);                                                               \
+        break;
+                switch (LW_) {
+                    GBK_SMSSP(8)
+                    GBK_SMSSP(16)
+                    GBK_SMSSP(32)
+                    GBK_SMSSP(64)
+                }
+#undef GBK_SMSSP
+                GBK_LAUNCH_CHECK();
+            }
+            GBK_HIP(hipMemcpyAsync(hbuf_, L_.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+            GBK_HIP(hipMemcpyAsync(hbuf_ + 2, L_.changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
+            GBK_HIP(hipStreamSynchronize(st_));
+            nq_ = hbuf_[0];
+            changed = *(unsigned long long*)(hbuf_ + 2);
+            std::swap(qa_, qb_);
+            ++round_;
+        }
+        (void)seeded;
+        for (int q = 0; q < k_; ++q)
+            if (((occupied_ >> q) & 1ull) && !((changed >> q) & 1ull)) finished.push_back(q);
+        return GBK_SUCCESS;
+    }
+    int extract(int slot, void* out_dev) override {
+        hipLaunchKernelGGL(k_extract_col<int32_t>, grid_b(n_), dim3(256), 0, st_, L_.D, n_, k_, slot, (int32_t*)out_dev);
+        GBK_LAUNCH_CHECK();
+        return GBK_SUCCESS;
+    }
+    void release(int slot) override { occupied_ &= ~(1ull << slot); }
+
+  private:
+    gbk_matrix A_;
+    hipStream_t st_;
+    int64_t n_;
+    int k_;
+    void* ws_ = nullptr;
+    int32_t* seeds_ = nullptr;
+    int32_t* hbuf_ = nullptr;
+    size_t ws_bytes_ = 0;
+    MsspLayout L_{};
+    int32_t *qa_ = nullptr, *qb_ = nullptr;
+    int64_t nq_ = 0;
+    int32_t round_ = 0;
+    int LW_ = 32;
+    uint64_t occupied_ = 0;
+    std::vector<int32_t> pend_src_, pend_slot_;
+};
+
+}  // namespace
+
+int make_ppr_engine(gbk_matrix A, hipStream_t st, int k, double damping, double tol, int itermax,
+                    std::unique_ptr<Engine>* out) {
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "PageRank engine takes 1..64 slots");
+    auto e = std::make_unique<PprEngine>(A, st, k, damping, tol, itermax);
+    GBK_TRY(e->init());
+    *out = std::move(e);
+    return GBK_SUCCESS;
+}
+
+int make_sssp_engine(gbk_matrix A, hipStream_t st, int k, std::unique_ptr<Engine>* out) {
+    if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "SSSP engine takes 1..64 slots");
+    auto e = std::make_unique<SsspEngine>(A, st, k);
+    GBK_TRY(e->init());
+    *out = std::move(e);
     return GBK_SUCCESS;
 }
 

@@ -393,6 +393,45 @@ int gbk_ppr_multi(gbk_matrix A, const int64_t* sources, int k, float* ranks, int
 int gbk_sssp_multi(gbk_matrix A, const int64_t* sources, int k, int32_t* dist, int64_t* stats);
 
 /* ------------------------------------------------------------------------ */
+/* Query server: continuous batching of BFS / personalized PageRank / SSSP
+ * queries on device-resident graphs.  No reference counterpart (the
+ * reference answers one query per driver call); each query's result equals
+ * the corresponding single-query call: gbk_bfs levels, gbk_ppr_multi ranks,
+ * gbk_sssp distances.                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef enum { GBK_QUERY_BFS = 0, GBK_QUERY_PPR = 1, GBK_QUERY_SSSP = 2 } gbk_query_kind;
+typedef struct gbk_server_s* gbk_server;
+
+/* hbm_budget_bytes bounds the HBM the server's engines and in-flight query
+ * results may hold (the graphs themselves are not counted).  ppr_slots and
+ * sssp_slots (1..64) are the batch widths of those engines (BFS uses 64).
+ * keep_results != 0 keeps every result vector for gbk_server_result. */
+int gbk_server_new(gbk_server* s, int64_t hbm_budget_bytes, int ppr_slots, int sssp_slots,
+                   double ppr_damping, double ppr_tol, int ppr_itermax, int keep_results);
+int gbk_server_free(gbk_server s);
+/* Registers a graph: its BFS matrix (any type, read as a pattern), its FP32
+ * PageRank transition matrix and its INT32 SSSP weight matrix; a NULL matrix
+ * means the graph takes no queries of that kind.  All must have one size. */
+int gbk_server_add_graph(gbk_server s, gbk_matrix bfs_A, gbk_matrix ppr_A, gbk_matrix sssp_A, int* graph_id);
+/* Queues a query that arrives arrival_s seconds after gbk_server_run starts. */
+int gbk_server_submit(gbk_server s, int graph_id, int kind, int64_t source, double arrival_s, int64_t* query_id);
+/* Serves every queued query (one worker thread and stream per graph and
+ * kind), admitting each when it has arrived, its engine has a free slot and
+ * the budget holds its footprint; returns when all have completed. */
+int gbk_server_run(gbk_server s);
+/* times: [0] arrival, [1] admission, [2] completion (seconds from the start
+ * of gbk_server_run), [3] checksum (sum of the reached entries); info:
+ * [0] reached entries, [1] batch occupancy at admission, [2] slot, [3] status. */
+int gbk_server_query_info(gbk_server s, int64_t query_id, double* times, int64_t* info);
+/* Copies a kept result: n int32 levels (BFS), n float ranks (PPR) or n int32
+ * distances (SSSP). */
+int gbk_server_result(gbk_server s, int64_t query_id, void* host_out);
+/* stats: [0] wall seconds of the last run, [1] queries, [2] peak bytes of the
+ * budget in use, [3] engines created, [4] most engines alive at once. */
+int gbk_server_stats(gbk_server s, double* stats);
+
+/* ------------------------------------------------------------------------ */
 /* Seeded graph generation (host) — replaces the unseeded
  * GraphGenerationAPI::RMAT / random_uniform
  * (src/helpers/graph_generation/graph_generation.hpp:37-190).
