@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--sigma", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="served measurement on the full 8-graph set (R-MAT 18..24 + ER 22, 20,000 queries)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed loop, copy this many MiB device-to-device on the library stream "
                          "(a kernel of known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
@@ -150,6 +153,12 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(O, rp, col, vals, x_np, algo_bytes, args.cpu_seconds)
 
+    batched = served = None
+    if not args.no_batched:
+        A.free()
+        batched = batched_bench(g, rp, col, n, args.scale)
+        served = served_bench(g, full=args.serve_full)
+
     out = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -175,8 +184,126 @@ def main():
                      "kernel": KERNELS.get(fmt_name.rstrip("0123456789"), "SpMV"),
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
+
+
+def _sources(rp, k, seed):
+    cand = np.flatnonzero(np.diff(rp) > 0)
+    return np.random.default_rng(seed).choice(cand, size=k, replace=False)
+
+
+def batched_bench(g, rp, col, n, scale):
+    """Batched queries on the headline graph: 64-source BFS as one boolean SpMM
+    traversal vs 64 single-source BFS; 32 personalized PageRanks (one-hot
+    starts, L1 < 1e-6) as one SpMM iteration vs 32 single-vector runs."""
+    res = {"graph": f"R-MAT scale {scale}, edge factor 16 (directed)"}
+    Ab = g.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    src = _sources(rp, 64, 7)
+    g.bfs(Ab, int(src[0]))  # warm-up
+    t0 = time.perf_counter()
+    single = [g.bfs(Ab, int(s))[0] for s in src]
+    t_single = time.perf_counter() - t0
+    g.msbfs(Ab, src[:2])
+    t0 = time.perf_counter()
+    lv, st = g.msbfs(Ab, src)
+    t_multi = time.perf_counter() - t0
+    same = all(np.array_equal(lv[i], single[i]) for i in range(len(src)))
+    res["bfs64"] = {"single_qps": round(64 / t_single, 2), "batched_qps": round(64 / t_multi, 2),
+                    "speedup": round(t_single / t_multi, 2), "levels_identical": bool(same),
+                    "batched_ms": round(t_multi * 1e3, 2), "single_ms_each": round(t_single / 64 * 1e3, 3)}
+    log(f"batched BFS x64: {t_multi * 1e3:.1f} ms vs {t_single * 1e3:.1f} ms single ({t_single / t_multi:.1f}x), "
+        f"identical={same}")
+    Ab.free()
+    Ap = g.Matrix.from_csr(rp, col, g.inv_coldeg_f32(n, col))
+    srcp = _sources(rp, 32, 8)
+    g.ppr_multi(Ap, srcp[:1], itermax=100, tol=1e-6)
+    t0 = time.perf_counter()
+    singles = [g.ppr_multi(Ap, [int(s)], itermax=100, tol=1e-6) for s in srcp]
+    t_single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r, its = g.ppr_multi(Ap, srcp, itermax=100, tol=1e-6)
+    t_multi = time.perf_counter() - t0
+    l1 = max(float(np.abs(r[i].astype(np.float64) - singles[i][0][0]).sum()) for i in range(len(srcp)))
+    res["ppr32"] = {"single_qps": round(32 / t_single, 2), "batched_qps": round(32 / t_multi, 2),
+                    "speedup": round(t_single / t_multi, 2), "max_l1_batched_vs_single": l1,
+                    "iterations": [int(v) for v in its[:4]], "batched_ms": round(t_multi * 1e3, 2)}
+    log(f"batched PPR x32: {t_multi * 1e3:.1f} ms vs {t_single * 1e3:.1f} ms single ({t_single / t_multi:.1f}x), "
+        f"max L1 batched vs single {l1:.1e}")
+    Ap.free()
+    return res
+
+
+def served_bench(g, full=False, seed=11):
+    """Poisson stream of BFS (50%), personalized PageRank (30%) and SSSP (20%)
+    queries over device-resident graphs, served with continuous batching;
+    p50/p99 latency and q/s, beside the plain per-query GPU path (each query
+    run alone by the single-query drivers) on a sample of the same mix."""
+    specs = ([("rmat", s) for s in range(18, 25)] + [("er", 22)]) if full else \
+        [("rmat", 18), ("rmat", 19), ("rmat", 20), ("er", 18)]
+    nq = 20000 if full else 3000
+    graphs = []
+    t0 = time.perf_counter()
+    for i, (kind, sc) in enumerate(specs):
+        if kind == "rmat":
+            rp, col = g.gen_rmat(sc, 16, seed=100 + i, symmetric=True)
+        else:
+            rp, col = g.gen_uniform(sc, 16, seed=100 + i, symmetric=True)
+        n = len(rp) - 1
+        mb = g.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+        mp = g.Matrix.from_csr(rp, col, g.inv_coldeg_f32(n, col))
+        ms = g.Matrix.from_csr(rp, col, g.sym_weights_i32(rp, col, 200 + i, 1, 255))
+        graphs.append({"name": f"{kind}{sc}", "rp": rp, "m": (mb, mp, ms), "n": n})
+    log(f"serving graphs built in {time.perf_counter() - t0:.1f} s: " +
+        ", ".join(f"{G['name']} n={G['n']} nnz={G['m'][0].nnz}" for G in graphs))
+    rng = np.random.default_rng(seed)
+    kinds = rng.choice([g.QUERY_BFS, g.QUERY_PPR, g.QUERY_SSSP], size=nq, p=[0.5, 0.3, 0.2])
+    gsel = rng.integers(0, len(graphs), size=nq)
+    srcs = [int(_sources(graphs[gi]["rp"], 1, 1000 + i)[0]) for i, gi in enumerate(gsel)]
+
+    # plain per-query path on a sample: every query alone through the single-query drivers
+    sample = min(150, nq)
+    t0 = time.perf_counter()
+    for i in range(sample):
+        G = graphs[gsel[i]]
+        if kinds[i] == g.QUERY_BFS:
+            g.bfs(G["m"][0], srcs[i])
+        elif kinds[i] == g.QUERY_PPR:
+            g.ppr_multi(G["m"][1], [srcs[i]], itermax=100, tol=1e-6)
+        else:
+            g.sssp(G["m"][2], srcs[i])
+    plain_qps = sample / (time.perf_counter() - t0)
+
+    def run(rate):
+        srv = g.Server(64 << 30, ppr_slots=32, sssp_slots=32, tol=1e-6, itermax=100, keep_results=False)
+        gids = [srv.add_graph(*G["m"]) for G in graphs]
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=nq)) if rate else np.zeros(nq)
+        qids = [srv.submit(gids[gsel[i]], int(kinds[i]), srcs[i], float(arr[i])) for i in range(nq)]
+        srv.run()
+        st = srv.stats()
+        lat = np.array([srv.info(q)["done"] - srv.info(q)["arrival"] for q in qids])
+        occ = np.array([srv.info(q)["occupancy"] for q in qids])
+        return st, lat, occ
+
+    st, lat, occ = run(None)  # everything at t = 0: capacity
+    cap_qps = nq / st["wall_s"]
+    rate = 0.5 * cap_qps
+    st2, lat2, occ2 = run(rate)
+    res = {"graphs": [G["name"] for G in graphs], "queries": nq, "mix": "50% BFS, 30% PPR (L1 1e-6), 20% SSSP",
+           "plain_per_query_qps": round(plain_qps, 1), "plain_sample": sample,
+           "capacity_qps": round(cap_qps, 1), "speedup_vs_plain": round(cap_qps / plain_qps, 2),
+           "poisson_rate_qps": round(rate, 1), "served_qps": round(nq / st2["wall_s"], 1),
+           "p50_ms": round(float(np.percentile(lat2, 50)) * 1e3, 2),
+           "p99_ms": round(float(np.percentile(lat2, 99)) * 1e3, 2),
+           "mean_batch_occupancy": round(float(occ2.mean()), 1), "peak_hbm_bytes": st2["peak_bytes"]}
+    log(f"served: capacity {cap_qps:.0f} q/s ({cap_qps / plain_qps:.1f}x plain {plain_qps:.0f} q/s); "
+        f"Poisson {rate:.0f} q/s -> p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms")
+    for G in graphs:
+        for m in G["m"]:
+            m.free()
+    return res
 
 
 def cpu_baseline(O, rp, col, vals, x, algo_bytes, seconds):

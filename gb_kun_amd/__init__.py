@@ -535,10 +535,10 @@ class Server:
         return out
 
     def stats(self):
-        st = (C.c_double * 5)()
+        st = (C.c_double * 6)()
         check(lib.gbk_server_stats(self.h, st))
         return {"wall_s": st[0], "queries": int(st[1]), "peak_bytes": int(st[2]), "engines": int(st[3]),
-                "max_engines_live": int(st[4])}
+                "max_engines_live": int(st[4]), "largest_engine_bytes": int(st[5])}
 
     def free(self):
         if self.h:

@@ -106,6 +106,8 @@ struct gbk_server_s {
     std::mutex mu;
     int64_t used = 0, peak = 0;
     int engines_made = 0;
+    std::atomic<int> waiters{0};  // workers waiting for budget to create an engine
+    int64_t largest_engine = 0;   // largest engine reservation (workspace estimate + one result)
     int max_engines_live = 0, engines_live = 0;
     double wall = 0.0;
     int first_error = GBK_SUCCESS;
@@ -165,6 +167,8 @@ void worker(gbk_server s, int gi, int kind, std::vector<int64_t> qids, clk::time
     std::vector<void*> rbuf(k, nullptr);
     std::vector<int64_t> slot_q(k, -1);
     int occupied = 0;
+    int held = 0;  // result footprints reserved from the budget (one stays with the engine)
+    bool waiting = false;
     size_t next = 0;
     double* dsum = nullptr;
     unsigned long long* dcnt = nullptr;
@@ -175,24 +179,39 @@ void worker(gbk_server s, int gi, int kind, std::vector<int64_t> qids, clk::time
         const double t = now();
         if (!eng && next < qids.size() && s->queries[qids[next]].arrival <= t) {
             const int64_t est = engine_estimate(A, kind, k);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->largest_engine = std::max(s->largest_engine, est + (int64_t)rbytes);
+            }
             if (est + (int64_t)rbytes > s->budget) {
                 rc = fail(GBK_OUT_OF_MEMORY, "a query on graph %d needs %lld bytes of HBM, over the server budget", gi,
                           (long long)(est + (int64_t)rbytes));
                 break;
             }
-            if (!try_reserve(s, est)) {
+            // an engine is created with room for one result, so that every
+            // engine that holds memory can always make progress
+            if (!try_reserve(s, est + (int64_t)rbytes)) {
+                if (!waiting) {
+                    waiting = true;
+                    ++s->waiters;
+                }
                 std::this_thread::sleep_for(std::chrono::microseconds(100));
                 continue;
+            }
+            if (waiting) {
+                waiting = false;
+                --s->waiters;
             }
             if (kind == GBK_QUERY_BFS) rc = make_bfs_engine(A, st, &eng);
             else if (kind == GBK_QUERY_PPR) rc = make_ppr_engine(A, st, k, s->damping, s->tol, s->itermax, &eng);
             else rc = make_sssp_engine(A, st, k, &eng);
             if (rc != GBK_SUCCESS) {
-                give_back(s, est);
+                give_back(s, est + (int64_t)rbytes);
                 break;
             }
             eng_bytes = (int64_t)eng->bytes();
             give_back(s, est);
+            held = 1;
             {
                 // the real workspace is held whatever the estimate was
                 std::lock_guard<std::mutex> lk(s->mu);
@@ -205,7 +224,10 @@ void worker(gbk_server s, int gi, int kind, std::vector<int64_t> qids, clk::time
         }
         // admit arrived queries into free slots while the budget holds them
         while (eng && next < qids.size() && occupied < k && s->queries[qids[next]].arrival <= now()) {
-            if (!try_reserve(s, (int64_t)rbytes)) break;
+            if (occupied >= held) {
+                if (!try_reserve(s, (int64_t)rbytes)) break;
+                ++held;
+            }
             int slot = 0;
             while (slot_q[slot] >= 0) ++slot;
             Query& q = s->queries[qids[next]];
@@ -223,6 +245,19 @@ void worker(gbk_server s, int gi, int kind, std::vector<int64_t> qids, clk::time
         if (rc != GBK_SUCCESS) break;
         if (occupied == 0) {
             if (eng && next >= qids.size()) break;
+            if (eng && s->waiters.load() > 0) {
+                // idle while another worker waits for memory: hand the engine's memory back
+                (void)hipStreamSynchronize(st);
+                eng.reset();
+                give_back(s, eng_bytes + (int64_t)held * (int64_t)rbytes);
+                eng_bytes = 0;
+                held = 0;
+                {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    --s->engines_live;
+                }
+                continue;
+            }
             // wait for the next arrival (or for budget)
             double wait = next < qids.size() ? s->queries[qids[next]].arrival - now() : 0.0;
             wait = std::min(std::max(wait, 0.0), 0.0002);
@@ -268,9 +303,13 @@ void worker(gbk_server s, int gi, int kind, std::vector<int64_t> qids, clk::time
             eng->release(slot);
             slot_q[slot] = -1;
             --occupied;
-            give_back(s, (int64_t)rbytes);
+            if (held > 1) {
+                give_back(s, (int64_t)rbytes);
+                --held;
+            }
         }
     }
+    if (waiting) --s->waiters;
     if (rc != GBK_SUCCESS) {
         record_error(rc);
         for (int slot = 0; slot < k; ++slot)
@@ -282,7 +321,7 @@ void worker(gbk_server s, int gi, int kind, std::vector<int64_t> qids, clk::time
     dev_free(dsum);
     dev_free(dcnt);
     if (eng_bytes) {
-        give_back(s, eng_bytes);
+        give_back(s, eng_bytes + (int64_t)held * (int64_t)rbytes);
         std::lock_guard<std::mutex> lk(s->mu);
         --s->engines_live;
     }
@@ -374,6 +413,7 @@ extern "C" int gbk_server_run(gbk_server s) {
                          [&](int64_t a, int64_t b) { return s->queries[a].arrival < s->queries[b].arrival; });
     s->first_error = GBK_SUCCESS;
     s->first_error_msg[0] = 0;
+    s->waiters = 0;
     s->used = 0;
     s->peak = 0;
     const auto t0 = clk::now();
@@ -420,5 +460,6 @@ extern "C" int gbk_server_stats(gbk_server s, double* stats) {
     stats[2] = (double)s->peak;
     stats[3] = (double)s->engines_made;
     stats[4] = (double)s->max_engines_live;
+    stats[5] = (double)s->largest_engine;
     return GBK_SUCCESS;
 }

@@ -427,8 +427,9 @@ int gbk_server_query_info(gbk_server s, int64_t query_id, double* times, int64_t
 /* Copies a kept result: n int32 levels (BFS), n float ranks (PPR) or n int32
  * distances (SSSP). */
 int gbk_server_result(gbk_server s, int64_t query_id, void* host_out);
-/* stats: [0] wall seconds of the last run, [1] queries, [2] peak bytes of the
- * budget in use, [3] engines created, [4] most engines alive at once. */
+/* stats (6 entries): [0] wall seconds of the last run, [1] queries, [2] peak
+ * bytes of the budget in use, [3] engines created, [4] most engines alive at
+ * once, [5] largest single engine reservation (workspace + one result). */
 int gbk_server_stats(gbk_server s, double* stats);
 
 /* ------------------------------------------------------------------------ */
