@@ -7,9 +7,22 @@
  * by __graft_entry__.smoke() and by bench.py's cpu_baseline leg only.
  *
  * Every function names the reference file:line it restates
- * (afanasyev-ilya/GB_Kun).  Pinning of each function is listed in
- * oracle/README.md (golden vectors written by the compiled reference, closed
- * forms, brute force on tiny inputs).
+ * (afanasyev-ilya/GB_Kun).  Each is pinned by tests/test_oracle.py before it
+ * is trusted as a checker:
+ *   orc_spmv            golden vectors of the compiled reference (fp32 bits,
+ *                       int32, masks, accumulators, the trailing-row defect),
+ *                       dense brute force, int32 min.plus saturation
+ *   orc_spmv_f64_of_f32 dense FP64 product on exactly representable data
+ *   orc_balance_rows +
+ *   orc_spmv_plus_times_f32_omp   reference fp32 bits on the golden graph
+ *   orc_spmspv          golden SpMSpV vectors of the compiled reference
+ *   orc_bfs             golden BFS levels; |level(u) - level(v)| <= 1 on edges
+ *   orc_dijkstra_i32    golden SSSP distances; triangle inequality / tree
+ *   orc_dijkstra_f32    equals orc_dijkstra_i32 on integer weights
+ *   orc_cc              golden CC labels; constant on edges, component minimum
+ *   orc_pagerank_f64    golden PageRank of the compiled reference; fixed point
+ *   orc_ppr_f64         closed-form linear solve; uniform teleport reduces it
+ *                       to orc_pagerank_f64
  *
  * Conventions: CSR with int64 row_ptr[n+1] and int32 col[nnz]; values and
  * vectors are arrays of one element type given by `type`

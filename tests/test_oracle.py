@@ -265,3 +265,49 @@ def test_ppr_closed_form(oracle, gbk):
     r_u, _ = oracle.ppr_f64(rp, col, val, p=np.full(n, 1.0 / n), itermax=300)
     r_g, _ = oracle.pagerank_f64(rp, col, itermax=300)
     assert np.abs(r_u - r_g).sum() < 1e-6  # fp32 1/coldeg values vs exact division
+
+
+def test_spmv_f64_closed_form(oracle):
+    """The FP64 restatement equals the dense FP64 product of the same fp32 data
+    (summation order aside: integers below 2^24 make every partial sum exact)."""
+    rp, col = _rand_csr(9, 53, 700)
+    n = 53
+    rng = np.random.default_rng(10)
+    vals = rng.integers(1, 1000, len(col)).astype(np.float32)
+    x = rng.integers(1, 1000, n).astype(np.float32)
+    D = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    np.add.at(D, (rows, col), vals.astype(np.float64))
+    y, absum = oracle.spmv_f64(rp, col, vals, x)
+    assert np.array_equal(y, D @ x.astype(np.float64))
+    assert np.array_equal(absum, np.abs(D) @ np.abs(x.astype(np.float64)))
+
+
+def test_cpu_baseline_kernel_matches_reference_bits(oracle):
+    """The timed OpenMP baseline (the reference's row partition and per-row left
+    fold) reproduces the compiled reference's fp32 CSR result bit for bit, and
+    its partition is the reference's: contiguous, covering, nnz-balanced."""
+    d = load("spmv_rmat9_f32")
+    rp, col, vals, x = d["rp"], d["col"], d["vals"], d["x"]
+    for t in (1, 3, 8):
+        first, last = oracle.balance_rows(rp, t)
+        assert first[0] == 0 and np.all(first[1:] == last[:-1])
+        y = np.zeros(len(rp) - 1, np.float32)
+        oracle.spmv_f32_omp(rp.astype(np.int64), col.astype(np.int32), vals, x, y, t, first, last)
+        covered = np.zeros(len(rp) - 1, bool)
+        for a, b in zip(first, last):
+            covered[a:b] = True
+        assert np.array_equal(y[covered], d["y_plus_times_csr"][covered])
+
+
+def test_dijkstra_f32_matches_integer_dijkstra(oracle, gbk):
+    """On integer weights below 2^24 the float Dijkstra is exact: it must equal
+    the int32 one (pinned by the reference's golden SSSP)."""
+    rp, col = gbk.gen_rmat(10, 8, seed=41, symmetric=True)
+    w = gbk.sym_weights_i32(rp, col, 42, 1, 255)
+    for s in (0, 5, 77):
+        di = oracle.dijkstra_i32(rp, col, w, s)
+        df = oracle.dijkstra_f32(rp, col, w.astype(np.float32), s)
+        reach = di != np.iinfo(np.int32).max
+        assert np.array_equal(df[reach].astype(np.int64), di[reach].astype(np.int64))
+        assert np.all(df[~reach] == np.finfo(np.float32).max)
