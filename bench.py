@@ -233,6 +233,47 @@ def batched_bench(g, rp, col, n, scale):
     log(f"batched PPR x32: {t_multi * 1e3:.1f} ms vs {t_single * 1e3:.1f} ms single ({t_single / t_multi:.1f}x), "
         f"max L1 batched vs single {l1:.1e}")
     Ap.free()
+    res["graphs512"] = small_graphs_bench(g)
+    return res
+
+
+def small_graphs_bench(g, count=512, seed=12):
+    """512 independent R-MAT graphs, scale uniform in [8, 16], edge factor in
+    [8, 32]: one BFS per graph by one persistent kernel replayed from a HIP
+    graph, vs the single-graph driver called once per graph."""
+    rng = np.random.default_rng(seed)
+    graphs, srcs = [], []
+    t0 = time.perf_counter()
+    for i in range(count):
+        sc, ef = int(rng.integers(8, 17)), int(rng.integers(8, 33))
+        rp, col = g.gen_rmat(sc, ef, seed=5000 + i)
+        graphs.append((rp, col))
+        srcs.append(int(_sources(rp, 1, 6000 + i)[0]))
+    batch = g.GraphBatch(graphs)
+    info = batch.info()
+    log(f"{count} graphs: {info['vertices']} vertices, {info['edges']} edges, built in {time.perf_counter() - t0:.1f} s")
+    batch.bfs(srcs)  # first replay
+    t0 = time.perf_counter()
+    levels, dev_ms = batch.bfs(srcs)
+    t_batch = time.perf_counter() - t0
+    mats = [g.Matrix.from_csr(rp, col, np.ones(len(col), np.float32)) for rp, col in graphs]
+    g.bfs(mats[0], srcs[0])
+    t0 = time.perf_counter()
+    same = True
+    for i, A in enumerate(mats):
+        lv, _ = g.bfs(A, srcs[i])
+        same &= bool(np.array_equal(lv, levels[i]))
+    t_single = time.perf_counter() - t0
+    for A in mats:
+        A.free()
+    res = {"graphs": count, "vertices": info["vertices"], "edges": info["edges"],
+           "persistent_workgroups": info["workgroups"], "batched_ms": round(t_batch * 1e3, 2),
+           "batched_device_ms": round(dev_ms, 3), "single_ms": round(t_single * 1e3, 2),
+           "speedup": round(t_single / t_batch, 2), "levels_identical": same,
+           "gteps_batched": round(info["edges"] / (dev_ms * 1e-3) / 1e9, 2)}
+    log(f"{count} graphs BFS: one HIP-graph launch {dev_ms:.2f} ms on device ({t_batch * 1e3:.1f} ms with copies) "
+        f"vs {t_single * 1e3:.0f} ms one call per graph ({t_single / t_batch:.0f}x), identical={same}")
+    batch.free()
     return res
 
 

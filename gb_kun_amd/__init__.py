@@ -137,6 +137,10 @@ _sig("gbk_msbfs_device", C.c_int, _vp, _pi64, C.c_int, _vp, _pi64)
 _sig("gbk_ppr_multi", C.c_int, _vp, _pi64, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double,
      C.POINTER(C.c_int))
 _sig("gbk_sssp_multi", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
+_sig("gbk_graph_batch_new", C.c_int, C.POINTER(_vp), C.c_int, _pi64, C.POINTER(_pi64), C.POINTER(_pi32))
+_sig("gbk_graph_batch_free", C.c_int, _vp)
+_sig("gbk_graph_batch_bfs", C.c_int, _vp, _pi64, _pi32, C.POINTER(C.c_double))
+_sig("gbk_graph_batch_info", C.c_int, _vp, _pi64)
 _sig("gbk_server_new", C.c_int, C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int)
 _sig("gbk_server_free", C.c_int, _vp)
 _sig("gbk_server_add_graph", C.c_int, _vp, _vp, _vp, _vp, C.POINTER(C.c_int))
@@ -491,6 +495,46 @@ def sssp_multi(A, sources):
     st = np.zeros(8, dtype=np.int64)
     check(lib.gbk_sssp_multi(A.h, _ptr(src, C.c_int64), len(src), _ptr(d, C.c_int32), _ptr(st, C.c_int64)))
     return d, st[:2]
+
+
+class GraphBatch:
+    """Many small graphs packed for one persistent, HIP-graph-replayed BFS launch."""
+
+    def __init__(self, graphs):
+        """graphs: list of (row_ptr int64[n+1], col int32[nnz]) with local ids."""
+        self._keep = [(np.ascontiguousarray(rp, dtype=np.int64), np.ascontiguousarray(c, dtype=np.int32))
+                      for rp, c in graphs]
+        G = len(self._keep)
+        self.nverts = np.array([len(rp) - 1 for rp, _ in self._keep], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.nverts)]).astype(np.int64)
+        rps = (_pi64 * G)(*[_ptr(rp, C.c_int64) for rp, _ in self._keep])
+        cols = (_pi32 * G)(*[_ptr(c, C.c_int32) for _, c in self._keep])
+        self.h = _vp()
+        check(lib.gbk_graph_batch_new(C.byref(self.h), G, _ptr(self.nverts, C.c_int64), rps, cols))
+
+    def bfs(self, sources):
+        """Levels of every graph (a list of int32 arrays) and the device time of the launch in ms."""
+        src = np.ascontiguousarray(sources, dtype=np.int64)
+        out = np.empty(int(self.offsets[-1]), dtype=np.int32)
+        ms = C.c_double(0)
+        check(lib.gbk_graph_batch_bfs(self.h, _ptr(src, C.c_int64), _ptr(out, C.c_int32), C.byref(ms)))
+        return [out[self.offsets[g]:self.offsets[g + 1]] for g in range(len(self.nverts))], ms.value
+
+    def info(self):
+        i = np.zeros(4, dtype=np.int64)
+        check(lib.gbk_graph_batch_info(self.h, _ptr(i, C.c_int64)))
+        return {"graphs": int(i[0]), "vertices": int(i[1]), "edges": int(i[2]), "workgroups": int(i[3])}
+
+    def free(self):
+        if self.h:
+            lib.gbk_graph_batch_free(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------ query server

@@ -392,6 +392,23 @@ int gbk_ppr_multi(gbk_matrix A, const int64_t* sources, int k, float* ranks, int
  * stats (may be NULL): [0] rounds, [1] edge relaxations. */
 int gbk_sssp_multi(gbk_matrix A, const int64_t* sources, int k, int32_t* dist, int64_t* stats);
 
+/* Many small graphs in one launch: BFS from one source in each of up to
+ * thousands of independent graphs (bfs_blast, algorithms/bfs/bfs.hpp:94-142,
+ * once per graph) by one persistent kernel captured in a HIP graph and
+ * replayed on every call.  Graphs hold 1..131072 vertices each. */
+typedef struct gbk_graph_batch_s* gbk_graph_batch;
+/* row_ptrs[g] (nverts[g]+1 int64) and col_ids[g] (int32 local ids) are the
+ * CSR of graph g. */
+int gbk_graph_batch_new(gbk_graph_batch* b, int ngraphs, const int64_t* nverts,
+                        const int64_t* const* row_ptrs, const int32_t* const* col_ids);
+int gbk_graph_batch_free(gbk_graph_batch b);
+/* levels (host, may be NULL): sum(nverts) int32, graph g's at the offset
+ * sum(nverts[0..g-1]); 1-based, 0 = unreached.  device_ms (may be NULL): time
+ * of the graph launch on the device. */
+int gbk_graph_batch_bfs(gbk_graph_batch b, const int64_t* sources, int32_t* levels, double* device_ms);
+/* info: [0] graphs, [1] vertices, [2] edges, [3] persistent workgroups. */
+int gbk_graph_batch_info(gbk_graph_batch b, int64_t* info);
+
 /* ------------------------------------------------------------------------ */
 /* Query server: continuous batching of BFS / personalized PageRank / SSSP
  * queries on device-resident graphs.  No reference counterpart (the

@@ -103,3 +103,16 @@ def test_error_codes_without_device(gbk):
     rc = gbk.lib.gbk_matrix_build_csr32(h, gbk._ptr(rp, C.c_int64), gbk._ptr(col, C.c_int32), gbk._ptr(vals), 2, 1)
     assert rc == gbk.INVALID_OBJECT
     gbk.lib.gbk_matrix_free(h)
+
+
+def test_graph_batch_rejects_oversized_graph(gbk):
+    """Validated on the host before any device call: a graph above the
+    per-graph vertex limit, and a column id out of range."""
+    n = (1 << 17) + 1
+    rp = np.zeros(n + 1, dtype=np.int64)
+    with pytest.raises(gbk.GbkError) as e:
+        gbk.GraphBatch([(rp, np.zeros(0, dtype=np.int32))])
+    assert e.value.code == gbk.INVALID_VALUE
+    with pytest.raises(gbk.GbkError) as e:
+        gbk.GraphBatch([(np.array([0, 1], dtype=np.int64), np.array([5], dtype=np.int32))])
+    assert e.value.code == gbk.INDEX_OUT_OF_BOUNDS
