@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--bfs-scale", type=int, default=23, help="scale of the symmetric R-MAT for BFS / SSSP GTEPS")
     ap.add_argument("--serve-full", action="store_true",
                     help="served measurement on the full 8-graph set (R-MAT 18..24 + ER 22, 20,000 queries)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -153,11 +154,12 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(O, rp, col, vals, x_np, algo_bytes, args.cpu_seconds)
 
-    batched = served = None
+    batched = served = traversal = None
     if not args.no_batched:
         A.free()
         batched = batched_bench(g, rp, col, n, args.scale)
         served = served_bench(g, full=args.serve_full)
+        traversal = traversal_bench(g, args.bfs_scale)
 
     out = {
         "metric": METRIC,
@@ -186,6 +188,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "bfs_sssp": traversal,
     }
     print(json.dumps(out), flush=True)
 
@@ -277,6 +280,53 @@ def small_graphs_bench(g, count=512, seed=12):
     return res
 
 
+def traversal_bench(g, scale, nsrc=64, seed=13):
+    """Direction-optimizing BFS and int32 min.plus SSSP on a symmetrized R-MAT
+    from 64 seeded non-isolated sources; GTEPS = undirected edges of the
+    reached component / time (harmonic mean over sources, as Graph500)."""
+    t0 = time.perf_counter()
+    rp, col = g.gen_rmat(scale, 16, seed=seed, symmetric=True)
+    n = len(rp) - 1
+    deg = np.diff(rp)
+    Ab = g.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    w = g.sym_weights_i32(rp, col, seed + 1, 1, 255)
+    As = g.Matrix.from_csr(rp, col, w)
+    log(f"traversal graph: symmetric R-MAT {scale}: n={n} stored nnz={len(col)} ({time.perf_counter() - t0:.1f} s)")
+    src = _sources(rp, nsrc, seed + 2)
+    g.bfs(Ab, int(src[0]))
+    teps, times, levels, dirs = [], [], [], [0, 0]
+    for s_ in src:
+        t1 = time.perf_counter()
+        lv, st = g.bfs(Ab, int(s_))
+        dt = time.perf_counter() - t1
+        m = float(deg[lv > 0].sum()) / 2.0
+        teps.append(m / dt)
+        times.append(dt)
+        levels.append(int(st[0]))
+        dirs[0] += int(st[1])
+        dirs[1] += int(st[2])
+    bfs = {"gteps_hmean": round(len(teps) / sum(1.0 / t for t in teps) / 1e9, 3),
+           "ms_median": round(float(np.median(times)) * 1e3, 3), "levels_median": int(np.median(levels)),
+           "push_steps": dirs[0], "pull_steps": dirs[1]}
+    g.sssp(As, int(src[0]))
+    teps, times = [], []
+    for s_ in src[:16]:
+        t1 = time.perf_counter()
+        d, st = g.sssp(As, int(s_))
+        dt = time.perf_counter() - t1
+        m = float(deg[d != np.iinfo(np.int32).max].sum()) / 2.0
+        teps.append(m / dt)
+        times.append(dt)
+    sssp = {"gteps_hmean": round(len(teps) / sum(1.0 / t for t in teps) / 1e9, 3),
+            "ms_median": round(float(np.median(times)) * 1e3, 3), "sources": 16}
+    log(f"BFS symmetric R-MAT {scale}: {bfs['gteps_hmean']} GTEPS (median {bfs['ms_median']} ms); "
+        f"SSSP {sssp['gteps_hmean']} GTEPS (median {sssp['ms_median']} ms)")
+    Ab.free()
+    As.free()
+    return {"graph": f"symmetric R-MAT scale {scale}, edge factor 16", "n": n, "stored_nnz": int(len(col)),
+            "sources": nsrc, "bfs": bfs, "sssp": sssp}
+
+
 def served_bench(g, full=False, seed=11):
     """Poisson stream of BFS (50%), personalized PageRank (30%) and SSSP (20%)
     queries over device-resident graphs, served with continuous batching;
@@ -330,17 +380,20 @@ def served_bench(g, full=False, seed=11):
 
     st, lat, occ = run(None)  # everything at t = 0: capacity
     cap_qps = nq / st["wall_s"]
-    rate = 0.5 * cap_qps
-    st2, lat2, occ2 = run(rate)
     res = {"graphs": [G["name"] for G in graphs], "queries": nq, "mix": "50% BFS, 30% PPR (L1 1e-6), 20% SSSP",
            "plain_per_query_qps": round(plain_qps, 1), "plain_sample": sample,
            "capacity_qps": round(cap_qps, 1), "speedup_vs_plain": round(cap_qps / plain_qps, 2),
-           "poisson_rate_qps": round(rate, 1), "served_qps": round(nq / st2["wall_s"], 1),
-           "p50_ms": round(float(np.percentile(lat2, 50)) * 1e3, 2),
-           "p99_ms": round(float(np.percentile(lat2, 99)) * 1e3, 2),
-           "mean_batch_occupancy": round(float(occ2.mean()), 1), "peak_hbm_bytes": st2["peak_bytes"]}
-    log(f"served: capacity {cap_qps:.0f} q/s ({cap_qps / plain_qps:.1f}x plain {plain_qps:.0f} q/s); "
-        f"Poisson {rate:.0f} q/s -> p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms")
+           "peak_hbm_bytes": st["peak_bytes"], "poisson": []}
+    for frac in (0.25, 0.5):
+        rate = frac * cap_qps
+        st2, lat2, occ2 = run(rate)
+        res["poisson"].append({"rate_qps": round(rate, 1), "served_qps": round(nq / st2["wall_s"], 1),
+                               "p50_ms": round(float(np.percentile(lat2, 50)) * 1e3, 2),
+                               "p99_ms": round(float(np.percentile(lat2, 99)) * 1e3, 2),
+                               "mean_batch_occupancy": round(float(occ2.mean()), 1)})
+        log(f"served: Poisson {rate:.0f} q/s -> p50 {res['poisson'][-1]['p50_ms']} ms "
+            f"p99 {res['poisson'][-1]['p99_ms']} ms")
+    log(f"served: capacity {cap_qps:.0f} q/s ({cap_qps / plain_qps:.1f}x plain {plain_qps:.0f} q/s)")
     for G in graphs:
         for m in G["m"]:
             m.free()
