@@ -24,6 +24,7 @@
 //          A (+.second) (t ./ d), d = max(coldeg / damping, 1 / damping),
 //          teleport = (1 - damping) / n + damping / n * sum of the sinks'
 //          ranks, with the SpMV on the merge-path kernel.
+#include <algorithm>
 #include <cfloat>
 #include <vector>
 
@@ -89,34 +90,149 @@ __global__ __launch_bounds__(256) void k_bfs_push(const int32_t* __restrict__ rp
 }
 
 // Pull: G lanes per vertex; an unvisited vertex joins the next frontier if
-// any in-neighbour is in the current one (level == cur).
+// any in-neighbour is in the current one.  The frontier is read from a
+// bitmap (n/8 bytes, L2-resident) rather than the level array; each lane
+// tests four in-edges per step with all loads issued first, and the group
+// stops as soon as one lane has found a parent.  Vertices with more than
+// BFS_HEAVY in-edges are left to k_bfs_pull_heavy.
+constexpr int BFS_HEAVY = 2048;
+
+__device__ __forceinline__ bool fb_test(const uint32_t* __restrict__ fb, int u) {
+    return (fb[u >> 5] >> (u & 31)) & 1u;
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void k_bfs_pull(const int32_t* __restrict__ crp, const int32_t* __restrict__ ccol,
-                                                  int64_t n, int32_t* __restrict__ level, int32_t cur,
-                                                  int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count) {
+                                                  int64_t n, int32_t* __restrict__ level,
+                                                  const uint32_t* __restrict__ fb, int32_t cur) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
     bool found = false;
     if (v < n && level[v] == 0) {
-        const int e1 = crp[v + 1];
-        for (int e = crp[v] + lane; e < e1; e += G) {
-            if (level[ccol[e]] == cur) {
-                found = true;
-                break;
-            }
-        }
-        if constexpr (G > 1) {
-            // any lane of the group
-            unsigned long long b = __ballot(found);
+        const int e0 = crp[v], e1 = crp[v + 1];
+        if (e1 - e0 <= BFS_HEAVY) {
             const int gbase = (threadIdx.x & 63) & ~(G - 1);
             const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
-            found = (b & gmask) != 0;
+            // group-uniform trip count, so every lane of the group takes part in each ballot
+            for (int base = e0; base < e1; base += 4 * G) {
+                int u[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int e = base + k * G + lane;
+                    u[k] = e < e1 ? ccol[e] : -1;
+                }
+                bool hit = false;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) hit |= (u[k] >= 0) && fb_test(fb, u[k]);
+                found |= hit;
+                // the whole group stops once any of its lanes has found a parent
+                if (__ballot(found) & gmask) break;
+            }
+            found = (__ballot(found) & gmask) != 0;
         }
     }
-    const bool take = found && lane == 0;
-    if (take) level[v] = cur + 1;
-    wave_append(take, (int32_t)v, nq_out, nq_count);
+    // the next frontier is compacted from the levels afterwards (no shared counter)
+    if (found && lane == 0) level[v] = cur + 1;
+}
+
+// Unvisited vertices with more than BFS_HEAVY in-edges: one workgroup each.
+__global__ __launch_bounds__(256) void k_bfs_pull_heavy(const int32_t* __restrict__ crp, const int32_t* __restrict__ ccol,
+                                                        const int32_t* __restrict__ heavy, int64_t nheavy,
+                                                        int32_t* __restrict__ level, const uint32_t* __restrict__ fb,
+                                                        int32_t cur) {
+    __shared__ int s_found;
+    const int64_t h = blockIdx.x;
+    if (h >= nheavy) return;
+    const int v = heavy[h];
+    if (level[v] != 0) return;  // uniform across the block
+    if (threadIdx.x == 0) s_found = 0;
+    __syncthreads();
+    const int e0 = crp[v], e1 = crp[v + 1];
+    for (int base = e0; base < e1; base += 4 * 256) {
+        int u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = base + k * 256 + (int)threadIdx.x;
+            u[k] = e < e1 ? ccol[e] : -1;
+        }
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hit |= (u[k] >= 0) && fb_test(fb, u[k]);
+        if (hit) s_found = 1;
+        __syncthreads();
+        const int f = s_found;
+        __syncthreads();
+        if (f) break;
+    }
+    if (threadIdx.x == 0 && s_found) level[v] = cur + 1;
+}
+
+// Compaction of the vertices whose level is `target` into a queue, in vertex
+// order: per-chunk counts, an exclusive scan of the counts, then a ballot /
+// prefix-sum write (no shared counter).
+constexpr int LC_CHUNK = 4096;
+
+__global__ __launch_bounds__(256) void k_lvl_count(const int32_t* __restrict__ level, int64_t n, int32_t target,
+                                                   int32_t* __restrict__ counts) {
+    __shared__ int s_c;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * LC_CHUNK;
+    int c = 0;
+    for (int i = threadIdx.x; i < LC_CHUNK; i += 256) {
+        const int64_t v = base + i;
+        if (v < n && level[v] == target) ++c;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_c, c);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s_c;
+}
+
+__global__ __launch_bounds__(256) void k_lvl_compact(const int32_t* __restrict__ level, int64_t n, int32_t target,
+                                                     const int32_t* __restrict__ offs, int32_t* __restrict__ out) {
+    __shared__ int s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * LC_CHUNK;
+    int o = offs[blockIdx.x];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int r = 0; r < LC_CHUNK; r += 256) {
+        const int64_t v = base + r + threadIdx.x;
+        const bool p = v < n && level[v] == target;
+        const unsigned long long b = __ballot(p);
+        if (lane == 0) s_w[wid] = __popcll(b);
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            if (w < wid) pre += s_w[w];
+            tot += s_w[w];
+        }
+        if (p) out[o + pre + __popcll(b & lanemask_lt64())] = (int32_t)v;
+        o += tot;
+        __syncthreads();
+    }
+}
+
+// The frontier bitmap of a queue.
+__global__ void k_fb_clear(uint32_t* __restrict__ fb, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) fb[i] = 0;
+}
+__global__ void k_fb_set(uint32_t* __restrict__ fb, const int32_t* __restrict__ q, int64_t nq) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        const int u = q[i];
+        atomicOr(fb + (u >> 5), 1u << (u & 31));
+    }
+}
+// Vertices with more than BFS_HEAVY in-edges.
+__global__ void k_heavy_list(const int32_t* __restrict__ crp, int64_t n, int32_t* __restrict__ out,
+                             int32_t* __restrict__ count) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = v < n && (crp[v + 1] - crp[v]) > BFS_HEAVY;
+    wave_append(take, (int32_t)v, out, count);
 }
 
 __global__ void k_set_level(int32_t* level, int64_t s, int32_t* q, int32_t* count) {
@@ -125,25 +241,31 @@ __global__ void k_set_level(int32_t* level, int64_t s, int32_t* q, int32_t* coun
     *count = 1;
 }
 
-__global__ void k_queue_edges(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
-                              unsigned long long* __restrict__ sum) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_queue_edges(const int32_t* __restrict__ rp, const int32_t* __restrict__ q,
+                                                     int64_t nq, unsigned long long* __restrict__ sum) {
+    __shared__ unsigned long long s_d;
+    if (threadIdx.x == 0) s_d = 0;
+    __syncthreads();
     unsigned long long d = 0;
-    if (i < nq) {
-        int u = q[i];
-        d = (unsigned long long)(rp[u + 1] - rp[u]);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
+        const int u = q[i];
+        d += (unsigned long long)(rp[u + 1] - rp[u]);
     }
-    // wave reduce then one atomic per wave
+    // wave reduce, one shared atomic per wave, one global atomic per block
     for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-    if ((threadIdx.x & 63) == 0 && d) atomicAdd(sum, d);
+    if ((threadIdx.x & 63) == 0 && d) atomicAdd(&s_d, d);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_d) atomicAdd(sum, s_d);
 }
 
 struct BfsScratch {
     int32_t* q[2];
-    int32_t* cnt;  // [2] counters
+    int32_t* cnt;  // [2] queue counters, [2] heavy count
     int32_t* deg;
     int32_t* offs;
     unsigned long long* esum;
+    uint32_t* fb;     // frontier bitmap
+    int32_t* heavy;   // vertices with more than BFS_HEAVY in-edges
 };
 
 static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats) {
@@ -152,20 +274,24 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
     const DevCSR& in = A->csc;
     hipStream_t st = stream();
     void* p;
-    // slot 5: two queues + degrees + offsets (n+1 each) + counters
+    // slot 5: two queues + degrees + offsets + heavy list (n+1 each) + counters + frontier bitmap
     const size_t qbytes = sizeof(int32_t) * (size_t)(n + 1);
-    GBK_TRY(scratch(5, 4 * qbytes + 64, &p));
+    const int64_t fwords = (n + 31) / 32;
+    GBK_TRY(scratch(5, 5 * qbytes + 64 + sizeof(uint32_t) * (size_t)fwords + 64, &p));
     BfsScratch s;
     s.q[0] = (int32_t*)p;
     s.q[1] = s.q[0] + (n + 1);
     s.deg = s.q[1] + (n + 1);
     s.offs = s.deg + (n + 1);
-    s.cnt = s.offs + (n + 1);
+    s.heavy = s.offs + (n + 1);
+    s.cnt = s.heavy + (n + 1);
     s.esum = (unsigned long long*)(((uintptr_t)(s.cnt + 4) + 15) & ~(uintptr_t)15);
+    s.fb = (uint32_t*)(((uintptr_t)(s.esum + 2) + 15) & ~(uintptr_t)15);
     GBK_TRY(fill_value(GBK_INT32, level, n, 0.0));
     hipLaunchKernelGGL(k_set_level, dim3(1), dim3(1), 0, st, level, source, s.q[0], s.cnt);
     GBK_LAUNCH_CHECK();
     int64_t nq = 1, m_u = A->nnz, levels = 0, pushes = 0, pulls = 0, traversed = 0;
+    int64_t nheavy = -1;  // listed on the first pull step
     int32_t cur = 1;
     bool pull = false;
     int qi = 0;
@@ -179,7 +305,8 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
         // out-edges of the frontier
         unsigned long long m_f = 0;
         GBK_TRY(fill_value(GBK_INT64, s.esum, 1, 0.0));
-        hipLaunchKernelGGL(k_queue_edges, grid_n(nq), dim3(256), 0, st, out.row_ptr, s.q[qi], nq, s.esum);
+        hipLaunchKernelGGL(k_queue_edges, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1024)), dim3(256), 0, st,
+                           out.row_ptr, s.q[qi], nq, s.esum);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipMemcpyAsync(&m_f, s.esum, sizeof(m_f), hipMemcpyDeviceToHost, st));
         GBK_HIP(hipStreamSynchronize(st));
@@ -192,10 +319,28 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
         GBK_TRY(fill_value(GBK_INT32, ncnt, 1, 0.0));
         if (pull) {
             ++pulls;
+            if (nheavy < 0) {
+                GBK_TRY(fill_value(GBK_INT32, s.cnt + 2, 1, 0.0));
+                hipLaunchKernelGGL(k_heavy_list, grid_n(n), dim3(256), 0, st, in.row_ptr, n, s.heavy, s.cnt + 2);
+                GBK_LAUNCH_CHECK();
+                int32_t hc = 0;
+                GBK_HIP(hipMemcpyAsync(&hc, s.cnt + 2, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                GBK_HIP(hipStreamSynchronize(st));
+                nheavy = hc;
+            }
+            hipLaunchKernelGGL(k_fb_clear, grid_n(fwords), dim3(256), 0, st, s.fb, fwords);
+            GBK_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_fb_set, grid_n(nq), dim3(256), 0, st, s.fb, s.q[qi], nq);
+            GBK_LAUNCH_CHECK();
+            if (nheavy > 0) {
+                hipLaunchKernelGGL(k_bfs_pull_heavy, dim3((unsigned)nheavy), dim3(256), 0, st, in.row_ptr, in.col,
+                                   s.heavy, nheavy, level, s.fb, cur);
+                GBK_LAUNCH_CHECK();
+            }
 #define GBK_PULL(GG)                                                                                  \
     case GG:                                                                                          \
-        hipLaunchKernelGGL(k_bfs_pull<GG>, grid_n(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, level, cur, \
-                           nxt, ncnt);                                                                \
+        hipLaunchKernelGGL(k_bfs_pull<GG>, grid_n(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, level, s.fb, \
+                           cur);                                                                      \
         break;
             switch (G) {
                 GBK_PULL(4)
@@ -205,6 +350,15 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
             }
 #undef GBK_PULL
             GBK_LAUNCH_CHECK();
+            // next frontier = vertices whose level became cur + 1, in vertex order
+            const int64_t nb = (n + LC_CHUNK - 1) / LC_CHUNK;
+            hipLaunchKernelGGL(k_lvl_count, dim3((unsigned)nb), dim3(256), 0, st, level, n, cur + 1, s.deg);
+            GBK_LAUNCH_CHECK();
+            int32_t total = 0;
+            GBK_TRY(exclusive_scan_i32(s.deg, s.offs, nb, &total));
+            hipLaunchKernelGGL(k_lvl_compact, dim3((unsigned)nb), dim3(256), 0, st, level, n, cur + 1, s.offs, nxt);
+            GBK_LAUNCH_CHECK();
+            GBK_HIP(hipMemcpyAsync(ncnt, &total, sizeof(int32_t), hipMemcpyHostToDevice, st));
         } else {
             ++pushes;
             hipLaunchKernelGGL(k_frontier_deg, grid_n(nq), dim3(256), 0, st, out.row_ptr, s.q[qi], nq, s.deg);

@@ -241,6 +241,47 @@ __global__ void k_msbfs_sources(const int32_t* __restrict__ src, int nsrc, uint6
 // Pull step: G lanes per vertex.  A vertex still missing some search bits
 // ORs the frontier words of its in-neighbours, stopping once every missing
 // bit is found.  Writes the new frontier word of every vertex (0 if none).
+// Block-level sums: `c` added and `m` ORed over the block, then one global
+// atomic each per block (kernels run on a capped grid, so a step costs a few
+// thousand global atomics, not one per wave).
+__device__ __forceinline__ void block_flush(unsigned long long c, unsigned long long m,
+                                            unsigned long long* __restrict__ counters,
+                                            unsigned long long* __restrict__ active) {
+    __shared__ unsigned long long s_c, s_m;
+    if (threadIdx.x == 0) {
+        s_c = 0;
+        s_m = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off, 64);
+        m |= __shfl_xor(m, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (c) atomicAdd(&s_c, c);
+        if (m) atomicOr(&s_m, m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_c && counters) atomicAdd(counters, s_c);
+        if (s_m && active) atomicOr(active, s_m);
+    }
+}
+
+constexpr int64_t MSBFS_GRID = 8192;
+
+static dim3 capped_grid(int64_t threads) {
+    int64_t b = (threads + 255) / 256;
+    if (b > MSBFS_GRID) b = MSBFS_GRID;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+// Pull step: G lanes per vertex.  A vertex still missing some search bits
+// ORs the frontier words of its in-neighbours, stopping once every missing
+// bit is found.  Writes the new frontier word of every vertex (0 if none).
+// Grid-stride over vertex groups; the stride is a multiple of G, so a group
+// keeps one vertex per pass.
 template <int G>
 __global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ crp, const int32_t* __restrict__ ccol,
                                                     int64_t n, uint64_t all, const uint64_t* __restrict__ F,
@@ -248,32 +289,47 @@ __global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ 
                                                     int32_t* __restrict__ lev, int32_t next_level,
                                                     unsigned long long* __restrict__ counters,
                                                     unsigned long long* __restrict__ active) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t v = gid / G;
-    const int lane = (int)(gid % G);
-    uint64_t found = 0;
-    uint64_t vis = all;
-    if (v < n) {
-        vis = V[v];
+    unsigned long long c = 0, m = 0;
+    const int64_t total = n * G;
+    for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+         gid += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = gid / G;
+        const int lane = (int)(gid % G);
+        uint64_t found = 0;
+        const uint64_t vis = V[v];
         const uint64_t want = all & ~vis;
         if (want) {
-            const int e1 = crp[v + 1];
-            int e = crp[v] + lane;
-            for (; e < e1; e += G) {
-                found |= F[ccol[e]] & want;
-                if (found == want) break;
+            // group-uniform trip count; the group stops once its lanes together hold every missing bit
+            const int e0 = crp[v], e1 = crp[v + 1];
+            for (int base = e0; base < e1; base += 4 * G) {
+                uint64_t f[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int e = base + k * G + lane;
+                    f[k] = e < e1 ? F[ccol[e]] : 0ull;
+                }
+                found |= (f[0] | f[1] | f[2] | f[3]) & want;
+                uint64_t gf = found;
+                if constexpr (G > 1) {
+#pragma unroll
+                    for (int off = G / 2; off > 0; off >>= 1)
+                        gf |= (uint64_t)__shfl_xor((unsigned long long)gf, off, G);
+                }
+                if (gf == want) break;
+            }
+            if constexpr (G > 1) {
+#pragma unroll
+                for (int off = G / 2; off > 0; off >>= 1)
+                    found |= (uint64_t)__shfl_xor((unsigned long long)found, off, G);
             }
         }
-    }
-    // OR across the G lanes of the group
-    if constexpr (G > 1) {
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) found |= (uint64_t)__shfl_xor((unsigned long long)found, off, G);
-    }
-    if (v < n) {
         if (lane == 0) {
             N[v] = found;
-            if (found) V[v] = vis | found;
+            if (found) {
+                V[v] = vis | found;
+                c += (unsigned long long)__popcll(found);
+                m |= found;
+            }
         }
         // level writes: lanes of the group split the 64 search bits
         if (found) {
@@ -281,38 +337,101 @@ __global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ 
                 if ((found >> q) & 1ull) lev[v * 64 + q] = next_level;
         }
     }
-    unsigned long long c = (v < n && lane == 0 && found) ? (unsigned long long)__popcll(found) : 0ull;
-    unsigned long long m = (v < n && lane == 0) ? (unsigned long long)found : 0ull;
+    block_flush(c, m, counters, active);
+}
+
+// Frontier statistics: vertices with a non-empty frontier word and the sum
+// of their out-degrees (the push / pull decision).
+__global__ __launch_bounds__(256) void k_msbfs_fstats(const int32_t* __restrict__ rp, const uint64_t* __restrict__ F,
+                                                      int64_t n, unsigned long long* __restrict__ nq,
+                                                      unsigned long long* __restrict__ edges) {
+    unsigned long long c = 0, d = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        if (F[v]) {
+            ++c;
+            d += (unsigned long long)(rp[v + 1] - rp[v]);
+        }
+    }
+    __shared__ unsigned long long s_c, s_d;
+    if (threadIdx.x == 0) {
+        s_c = 0;
+        s_d = 0;
+    }
+    __syncthreads();
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         c += __shfl_xor(c, off, 64);
-        m |= __shfl_xor(m, off, 64);
+        d += __shfl_xor(d, off, 64);
     }
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(counters, c);
-    // searches that found a vertex in this step
-    if (active && (threadIdx.x & 63) == 0 && m) atomicOr(active, m);
+    if ((threadIdx.x & 63) == 0) {
+        if (c) atomicAdd(&s_c, c);
+        if (d) atomicAdd(&s_d, d);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_c) atomicAdd(nq, s_c);
+        if (s_d) atomicAdd(edges, s_d);
+    }
 }
 
-// Compacts the vertices with a non-empty frontier word into a queue and sums
-// their out-degrees (the push/pull decision and the push expansion use it).
-__global__ __launch_bounds__(256) void k_msbfs_queue(const int32_t* __restrict__ rp, const uint64_t* __restrict__ F,
-                                                     int64_t n, int32_t* __restrict__ q, int32_t* __restrict__ qn,
-                                                     unsigned long long* __restrict__ edges) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool take = v < n && F[v] != 0;
-    unsigned long long d = take ? (unsigned long long)(rp[v + 1] - rp[v]) : 0ull;
-    const unsigned long long b = __ballot(take);
-    const int lane = threadIdx.x & 63;
-    if (b) {
-        const int leader = __ffsll((long long)b) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(qn, (int)__popcll(b));
-        base = __shfl(base, leader, 64);
-        if (take) q[base + __popcll(b & lanemask_lt_b())] = (int32_t)v;
+// Queue of the vertices with a non-empty frontier word, in vertex order:
+// per-chunk counts, a scan, then a ballot / prefix-sum write.
+constexpr int FQ_CHUNK = 4096;
+
+__global__ __launch_bounds__(256) void k_fq_count(const uint64_t* __restrict__ F, int64_t n, int32_t* __restrict__ counts) {
+    __shared__ int s_c;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * FQ_CHUNK;
+    int c = 0;
+    for (int i = threadIdx.x; i < FQ_CHUNK; i += 256) {
+        const int64_t v = base + i;
+        if (v < n && F[v]) ++c;
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-    if (lane == 0 && d) atomicAdd(edges, d);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_c, c);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s_c;
+}
+
+__global__ __launch_bounds__(256) void k_fq_compact(const uint64_t* __restrict__ F, int64_t n,
+                                                    const int32_t* __restrict__ offs, int32_t* __restrict__ q) {
+    __shared__ int s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * FQ_CHUNK;
+    int o = offs[blockIdx.x];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int r = 0; r < FQ_CHUNK; r += 256) {
+        const int64_t v = base + r + threadIdx.x;
+        const bool p = v < n && F[v] != 0;
+        const unsigned long long b = __ballot(p);
+        if (lane == 0) s_w[wid] = __popcll(b);
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            if (w < wid) pre += s_w[w];
+            tot += s_w[w];
+        }
+        if (p) q[o + pre + __popcll(b & lanemask_lt_b())] = (int32_t)v;
+        o += tot;
+        __syncthreads();
+    }
+}
+
+// Frontier queue (vertices with a non-empty frontier word, ascending) built
+// without a shared counter; `counts` and `offs` need n / FQ_CHUNK + 1 entries.
+static int build_frontier_queue(const uint64_t* F, int64_t n, int32_t* q, int32_t* counts, int32_t* offs,
+                                void* partial, size_t part_bytes, hipStream_t st, int64_t* nq) {
+    const int64_t nb = (n + FQ_CHUNK - 1) / FQ_CHUNK;
+    hipLaunchKernelGGL(k_fq_count, dim3((unsigned)(nb > 0 ? nb : 1)), dim3(256), 0, st, F, n, counts);
+    GBK_LAUNCH_CHECK();
+    int32_t total = 0;
+    GBK_TRY(exclusive_scan_i32_on(counts, offs, nb, &total, st, partial, part_bytes));
+    hipLaunchKernelGGL(k_fq_compact, dim3((unsigned)(nb > 0 ? nb : 1)), dim3(256), 0, st, F, n, offs, q);
+    GBK_LAUNCH_CHECK();
+    *nq = total;
+    return GBK_SUCCESS;
 }
 
 __global__ void k_queue_degrees(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
@@ -349,9 +468,8 @@ __global__ __launch_bounds__(256) void k_msbfs_settle(int64_t n, uint64_t* __res
                                                       int32_t* __restrict__ lev, int32_t next_level,
                                                       unsigned long long* __restrict__ counters,
                                                       unsigned long long* __restrict__ active) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c = 0, m = 0;
-    if (v < n) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t vis = V[v];
         const uint64_t nw = N[v] & ~vis;
         N[v] = nw;
@@ -363,17 +481,11 @@ __global__ __launch_bounds__(256) void k_msbfs_settle(int64_t n, uint64_t* __res
                 lev[v * 64 + qb] = next_level;
                 b &= b - 1;
             }
-            c = (unsigned long long)__popcll(nw);
-            m = nw;
+            c += (unsigned long long)__popcll(nw);
+            m |= nw;
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        c += __shfl_xor(c, off, 64);
-        m |= __shfl_xor(m, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(counters, c);
-    if (active && (threadIdx.x & 63) == 0 && m) atomicOr(active, m);
+    block_flush(c, m, counters, active);
 }
 
 __global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
@@ -472,15 +584,14 @@ int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t w
     int32_t cur = 1;
     unsigned long long h[3];
     while (true) {
-        // frontier queue and its out-edge count
+        // frontier size and out-edge count decide the direction
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (uint64_t*)ctr, (int64_t)3);
         GBK_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_msbfs_queue, grid_of(n), dim3(256), 0, st, out.row_ptr, F, n, q, (int32_t*)(ctr + 2),
-                           ctr + 1);
+        hipLaunchKernelGGL(k_msbfs_fstats, capped_grid(n), dim3(256), 0, st, out.row_ptr, F, n, ctr + 2, ctr + 1);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         GBK_HIP(hipStreamSynchronize(st));
-        const int64_t nq = (int64_t)(h[2] & 0xffffffffull);
+        int64_t nq = (int64_t)h[2];
         const unsigned long long m_f = h[1];
         if (nq == 0) break;
         edges_scanned += (int64_t)m_f;
@@ -489,6 +600,7 @@ int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t w
         GBK_LAUNCH_CHECK();
         if (push) {
             ++pushes;
+            GBK_TRY(build_frontier_queue(F, n, q, deg, offs, partial, part_bytes, st, &nq));
             hipLaunchKernelGGL(k_zero_u64, grid_of(n), dim3(256), 0, st, N, n);
             GBK_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_queue_degrees, grid_of(nq), dim3(256), 0, st, out.row_ptr, q, nq, deg);
@@ -500,15 +612,15 @@ int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t w
                                    (int64_t)ne, F, V, N);
                 GBK_LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_msbfs_settle, grid_of(n), dim3(256), 0, st, n, V, N, lev, cur + 1, ctr,
+            hipLaunchKernelGGL(k_msbfs_settle, capped_grid(n), dim3(256), 0, st, n, V, N, lev, cur + 1, ctr,
                                (unsigned long long*)nullptr);
             GBK_LAUNCH_CHECK();
         } else {
             ++pulls;
 #define GBK_MSPULL(GG)                                                                                         \
     case GG:                                                                                                   \
-        hipLaunchKernelGGL(k_msbfs_pull<GG>, grid_of(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, all, F, V, \
-                           N, lev, cur + 1, ctr, (unsigned long long*)nullptr);                                \
+        hipLaunchKernelGGL(k_msbfs_pull<GG>, capped_grid(n * GG), dim3(256), 0, st, in.row_ptr, in.col, n, all, F, \
+                           V, N, lev, cur + 1, ctr, (unsigned long long*)nullptr);                             \
         break;
             switch (G) {
                 GBK_MSPULL(4)
@@ -652,20 +764,20 @@ class BfsEngine : public Engine {
             pend_slot_.clear();
         }
         if (!occupied_) return GBK_SUCCESS;
-        // frontier queue and its out-edges decide the direction
+        // frontier size and out-edge count decide the direction
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st_, (uint64_t*)ctr, (int64_t)4);
         GBK_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_msbfs_queue, grid_of(n_), dim3(256), 0, st_, out.row_ptr, L_.F, n_, L_.q,
-                           (int32_t*)(ctr + 2), ctr + 1);
+        hipLaunchKernelGGL(k_msbfs_fstats, capped_grid(n_), dim3(256), 0, st_, out.row_ptr, L_.F, n_, ctr + 2, ctr + 1);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipMemcpyAsync(hctr_, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
         GBK_HIP(hipStreamSynchronize(st_));
-        const int64_t nq = (int64_t)(hctr_[2] & 0xffffffffull);
+        int64_t nq = (int64_t)hctr_[2];
         const bool push = (double)hctr_[1] * 14.0 < (double)out.nnz;
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st_, (uint64_t*)ctr, (int64_t)4);
         GBK_LAUNCH_CHECK();
         if (nq > 0) {
             if (push) {
+                GBK_TRY(build_frontier_queue(L_.F, n_, L_.q, L_.deg, L_.offs, L_.partial, L_.part_bytes, st_, &nq));
                 hipLaunchKernelGGL(k_zero_u64, grid_of(n_), dim3(256), 0, st_, L_.N, n_);
                 GBK_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_queue_degrees, grid_of(nq), dim3(256), 0, st_, out.row_ptr, L_.q, nq, L_.deg);
@@ -677,13 +789,13 @@ class BfsEngine : public Engine {
                                        L_.offs, (int64_t)ne, L_.F, L_.V, L_.N);
                     GBK_LAUNCH_CHECK();
                 }
-                hipLaunchKernelGGL(k_msbfs_settle, grid_of(n_), dim3(256), 0, st_, n_, L_.V, L_.N, L_.lev, cur_ + 1,
+                hipLaunchKernelGGL(k_msbfs_settle, capped_grid(n_), dim3(256), 0, st_, n_, L_.V, L_.N, L_.lev, cur_ + 1,
                                    ctr, ctr + 3);
                 GBK_LAUNCH_CHECK();
             } else {
 #define GBK_SPULL(GG)                                                                                       \
     case GG:                                                                                                \
-        hipLaunchKernelGGL(k_msbfs_pull<GG>, grid_of(n_ * GG), dim3(256), 0, st_, in.row_ptr, in.col, n_,    \
+        hipLaunchKernelGGL(k_msbfs_pull<GG>, capped_grid(n_ * GG), dim3(256), 0, st_, in.row_ptr, in.col, n_, \
                            (uint64_t)occupied_, L_.F, L_.V, L_.N, L_.lev, cur_ + 1, ctr, ctr + 3);          \
         break;
                 switch (G_) {

@@ -209,3 +209,22 @@ def test_pagerank_vs_fp64(gbk, oracle, scale):
     assert it64 == it
     assert np.abs(r.astype(np.float64) - r64).sum() <= 1e-6
     assert abs(r.astype(np.float64).sum() - 1.0) < 1e-5
+
+
+def test_bfs_pull_heavy_vertex(gbk, oracle):
+    """Two hubs over the same 5000 leaves: the pull steps meet an unvisited hub
+    with more in-edges than the per-group limit (the one-workgroup-per-vertex
+    path), once with no parent in the frontier and once with 5000."""
+    from tests.graphs import csr_from_edges
+    L = 5000
+    h2 = L + 1
+    edges = []
+    for leaf in range(1, L + 1):
+        edges += [(0, leaf), (leaf, 0), (h2, leaf), (leaf, h2)]
+    rp, col = csr_from_edges(L + 2, edges)
+    A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    got, st = gbk.bfs(A, 0)
+    want = oracle.bfs(rp, col, 0)
+    assert np.array_equal(got, want)
+    assert got[0] == 1 and got[h2] == 3 and np.all(got[1:L + 1] == 2)
+    assert st[2] >= 1, st  # pull steps ran
