@@ -72,18 +72,15 @@ __global__ __launch_bounds__(256) void k_bfs_push(const int32_t* __restrict__ rp
                                                   const int32_t* __restrict__ offs, int64_t nedges,
                                                   int32_t* __restrict__ level, int32_t next_level,
                                                   int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count) {
+    __shared__ int32_t s_offs[EO_WIN + 1];
+    __shared__ int64_t s_i0;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = edge_owner(offs, nq, e, s_offs, &s_i0);
     bool take = false;
     int32_t v = 0;
     if (e < nedges) {
-        int64_t lo = 0, hi = nq - 1;
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if (offs[mid] <= e) lo = mid;
-            else hi = mid - 1;
-        }
-        const int u = q[lo];
-        v = col[rp[u] + (e - offs[lo])];
+        const int u = q[i];
+        v = col[rp[u] + (e - offs[i])];
         if (level[v] == 0) take = (atomicCAS(&level[v], 0, next_level) == 0);
     }
     wave_append(take, v, nq_out, nq_count);
@@ -415,29 +412,20 @@ __global__ __launch_bounds__(256) void k_sssp_relax(const int32_t* __restrict__ 
                                                     const T* __restrict__ w, const int32_t* __restrict__ q, int64_t nq,
                                                     const int32_t* __restrict__ offs, int64_t nedges,
                                                     T* __restrict__ dist, int32_t* __restrict__ inq,
-                                                    int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count,
                                                     int32_t round) {
+    __shared__ int32_t s_offs[EO_WIN + 1];
+    __shared__ int64_t s_i0;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool take = false;
-    int32_t v = 0;
+    const int64_t i = edge_owner(offs, nq, e, s_offs, &s_i0);
     if (e < nedges) {
-        int64_t lo = 0, hi = nq - 1;
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if (offs[mid] <= e) lo = mid;
-            else hi = mid - 1;
-        }
-        const int u = q[lo];
-        const int64_t k = rp[u] + (e - offs[lo]);
-        v = col[k];
-        const T du = dist[u];
-        const T nd = dist_add(du, w[k]);
-        if (nd < dist[v] && atomic_min_improve(dist + v, nd)) {
-            // enqueue v once per round
-            take = (atomicExch(inq + v, round) != round);
-        }
+        const int u = q[i];
+        const int64_t k = rp[u] + (e - offs[i]);
+        const int v = col[k];
+        const T nd = dist_add(dist[u], w[k]);
+        // an improved vertex is marked with the round; the next queue is
+        // compacted from the marks (no shared counter)
+        if (nd < dist[v] && atomic_min_improve(dist + v, nd)) inq[v] = round;
     }
-    wave_append(take, v, nq_out, nq_count);
 }
 
 template <typename T> __global__ void k_sssp_init(T* dist, int64_t n, int64_t s, T inf, int32_t* q, int32_t* count,
@@ -476,15 +464,21 @@ template <typename T> static int sssp_run(gbk_matrix A, int64_t source, T* dist_
         int32_t ne = 0;
         GBK_TRY(exclusive_scan_i32(deg, offs, nq, &ne));
         GBK_TRY(fill_value(GBK_INT32, cnt + (qi ^ 1), 1, 0.0));
+        int32_t c = 0;
         if (ne > 0) {
             hipLaunchKernelGGL((k_sssp_relax<T>), grid_n(ne), dim3(256), 0, st, m.row_ptr, m.col, (const T*)m.val,
-                               q[qi], nq, offs, (int64_t)ne, dist_dev, inq, q[qi ^ 1], cnt + (qi ^ 1), (int32_t)rounds);
+                               q[qi], nq, offs, (int64_t)ne, dist_dev, inq, (int32_t)rounds);
+            GBK_LAUNCH_CHECK();
+            // next queue: the vertices marked in this round, in vertex order
+            const int64_t nb = (n + LC_CHUNK - 1) / LC_CHUNK;
+            hipLaunchKernelGGL(k_lvl_count, dim3((unsigned)nb), dim3(256), 0, st, inq, n, (int32_t)rounds, deg);
+            GBK_LAUNCH_CHECK();
+            GBK_TRY(exclusive_scan_i32(deg, offs, nb, &c));
+            hipLaunchKernelGGL(k_lvl_compact, dim3((unsigned)nb), dim3(256), 0, st, inq, n, (int32_t)rounds, offs,
+                               q[qi ^ 1]);
             GBK_LAUNCH_CHECK();
         }
         relaxed += ne;
-        int32_t c = 0;
-        GBK_HIP(hipMemcpyAsync(&c, cnt + (qi ^ 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        GBK_HIP(hipStreamSynchronize(st));
         nq = c;
         qi ^= 1;
         ++rounds;

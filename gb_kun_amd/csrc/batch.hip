@@ -449,16 +449,13 @@ __global__ __launch_bounds__(256) void k_msbfs_push(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ offs, int64_t nedges,
                                                     const uint64_t* __restrict__ F, const uint64_t* __restrict__ V,
                                                     uint64_t* __restrict__ N) {
+    __shared__ int32_t s_offs[EO_WIN + 1];
+    __shared__ int64_t s_i0;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = edge_owner(offs, nq, e, s_offs, &s_i0);
     if (e >= nedges) return;
-    int64_t lo = 0, hi = nq - 1;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (offs[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    const int u = q[lo];
-    const int v = col[rp[u] + (e - offs[lo])];
+    const int u = q[i];
+    const int v = col[rp[u] + (e - offs[i])];
     const uint64_t bits = F[u] & ~V[v];
     if (bits) atomicOr((unsigned long long*)(N + v), (unsigned long long)bits);
 }

@@ -70,6 +70,53 @@ struct gbk_vector_s {
 
 namespace gbk {
 
+// Owner lookup for edge-balanced expansions over a queue: offs[0..nq] are the
+// exclusive prefix sums of the queue entries' degrees (offs[nq] = total), and
+// thread e of the grid handles edge e.  One thread binary-searches the global
+// offsets for the block's first edge; the block stages the next EO_WIN + 1
+// offsets in LDS and every thread searches there, falling back to the global
+// array when the block's edges span more entries than the window.  Every
+// thread of the block must call it (it synchronizes the block).
+constexpr int EO_WIN = 256;
+
+__device__ __forceinline__ int64_t edge_owner(const int32_t* __restrict__ offs, int64_t nq, int64_t e,
+                                              int32_t* s_offs, int64_t* s_i0) {
+    const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+    if (threadIdx.x == 0) {
+        int64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= e0) lo = mid;
+            else hi = mid - 1;
+        }
+        *s_i0 = lo;
+    }
+    __syncthreads();
+    const int64_t i0 = *s_i0;
+    const int64_t wn = (nq - i0 < EO_WIN) ? nq - i0 : EO_WIN;
+    for (int64_t j = threadIdx.x; j <= wn; j += blockDim.x) s_offs[j] = offs[i0 + j];
+    __syncthreads();
+    int64_t lo, hi;
+    if (e < s_offs[wn]) {
+        lo = 0;
+        hi = wn - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (s_offs[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        return i0 + lo;
+    }
+    lo = i0;
+    hi = nq - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // runtime.hip
 int ensure_init();
 hipStream_t stream();

@@ -16,8 +16,9 @@
 //
 // GPU form: the frontier's row lengths are prefix-summed, and the expansion
 // is edge-balanced: every thread owns one edge and finds its frontier entry
-// by binary search in the prefix sums, so hub vertices in the frontier are
-// spread over many waves.
+// in the prefix sums (edge_owner: one global search per block, then a search
+// in an LDS window), so hub vertices in the frontier are spread over many
+// waves.
 #include "gbk_objects.h"
 
 namespace gbk {
@@ -102,16 +103,12 @@ __global__ __launch_bounds__(256) void k_push(const int32_t* __restrict__ rp, co
                                               const T* __restrict__ val, const int32_t* __restrict__ x_ids,
                                               const T* __restrict__ x_vals, int64_t nv,
                                               const int32_t* __restrict__ offs, int64_t nedges, T* __restrict__ y) {
+    __shared__ int32_t s_offs[EO_WIN + 1];
+    __shared__ int64_t s_i0;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nedges) return;
     // frontier entry i with offs[i] <= e < offs[i+1]
-    int64_t lo = 0, hi = nv - 1;
-    while (lo < hi) {
-        int64_t mid = (lo + hi + 1) >> 1;
-        if (offs[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    const int64_t i = lo;
+    const int64_t i = edge_owner(offs, nv, e, s_offs, &s_i0);
+    if (e >= nedges) return;
     const int u = x_ids[i];
     const int64_t k = rp[u] + (e - offs[i]);
     const int d = col[k];
