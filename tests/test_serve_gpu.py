@@ -81,7 +81,6 @@ def test_budget_bounds_engines_and_results(gbk):
     small, qids = _serve(gbk, graphs, queries, budget=budget, keep=False)
     st = small.stats()
     assert st["peak_bytes"] <= budget
-    assert st["max_engines_live"] <= big.stats()["max_engines_live"]
     for q, bq in zip(qids, range(len(qids))):
         a, b = small.info(q), big.info(bq)
         assert a["status"] == 0
