@@ -203,17 +203,28 @@ def batched_bench(g, rp, col, n, scale):
     traversal vs 64 single-source BFS; 32 personalized PageRanks (one-hot
     starts, L1 < 1e-6) as one SpMM iteration vs 32 single-vector runs."""
     res = {"graph": f"R-MAT scale {scale}, edge factor 16 (directed)"}
+    from gb_kun_amd.device import DeviceArray
     Ab = g.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
     src = _sources(rp, 64, 7)
-    g.bfs(Ab, int(src[0]))  # warm-up
+    # levels stay in HBM on both sides (64 x n int32 each)
+    out_single = DeviceArray((64, n), np.int32)
+    out_multi = DeviceArray((64, n), np.int32)
+    g.bfs_device(Ab, int(src[0]), out_single.ptr)  # warm-up
+    g.synchronize()
     t0 = time.perf_counter()
-    single = [g.bfs(Ab, int(s))[0] for s in src]
+    for i, s_ in enumerate(src):
+        g.bfs_device(Ab, int(s_), out_single.ptr + 4 * n * i)
+    g.synchronize()
     t_single = time.perf_counter() - t0
-    g.msbfs(Ab, src[:2])
+    g.msbfs_device(Ab, src[:2], out_multi.ptr)
+    g.synchronize()
     t0 = time.perf_counter()
-    lv, st = g.msbfs(Ab, src)
+    g.msbfs_device(Ab, src, out_multi.ptr)
+    g.synchronize()
     t_multi = time.perf_counter() - t0
-    same = all(np.array_equal(lv[i], single[i]) for i in range(len(src)))
+    same = bool(np.array_equal(out_single.numpy(), out_multi.numpy()))
+    out_single.free()
+    out_multi.free()
     res["bfs64"] = {"single_qps": round(64 / t_single, 2), "batched_qps": round(64 / t_multi, 2),
                     "speedup": round(t_single / t_multi, 2), "levels_identical": bool(same),
                     "batched_ms": round(t_multi * 1e3, 2), "single_ms_each": round(t_single / 64 * 1e3, 3)}

@@ -435,6 +435,13 @@ def bfs(A, source):
     return lv, st[:4]
 
 
+def bfs_device(A, source, levels_ptr):
+    """BFS writing int32 levels to a device array of n entries; returns the stats."""
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_bfs_device(A.h, source, _vp(levels_ptr), _ptr(st, C.c_int64)))
+    return st[:4]
+
+
 def sssp(A, source):
     """min.plus SSSP distances in the matrix type (INT32: INT32_MAX unreached, FP32: FLT_MAX)."""
     d = np.empty(A.n, dtype=NP_TYPES[A.type])
@@ -474,6 +481,14 @@ def msbfs(A, sources):
     st = np.zeros(8, dtype=np.int64)
     check(lib.gbk_msbfs(A.h, _ptr(src, C.c_int64), len(src), _ptr(lv, C.c_int32), _ptr(st, C.c_int64)))
     return lv, st[:5]
+
+
+def msbfs_device(A, sources, levels_ptr):
+    """Multi-source BFS writing int32 levels[len(sources), n] to a device array; returns the stats."""
+    src = np.ascontiguousarray(sources, dtype=np.int64)
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_msbfs_device(A.h, _ptr(src, C.c_int64), len(src), _vp(levels_ptr), _ptr(st, C.c_int64)))
+    return st[:5]
 
 
 def ppr_multi(A, sources, itermax=100, damping=0.85, tol=1e-6):
