@@ -41,6 +41,9 @@ size_t msbfs_ws_bytes(int64_t n);
 int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t ws_bytes, int32_t* levels_out,
               int64_t* stats, hipStream_t st);
 
+size_t ppr_ws_bytes(gbk_matrix A, int k);
+size_t mssp_ws_bytes(gbk_matrix A, int k);
+
 // One (graph, query kind) batch of the query server: up to slots() queries in
 // flight, advanced together one level / iteration / round per step() on the
 // engine's stream.  A query admitted into a free slot joins at the next step;

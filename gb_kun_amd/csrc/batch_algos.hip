@@ -13,9 +13,10 @@
 // reference's PageRank with sinks.
 //
 // Multi-source SSSP (sssp_bellman_ford_blast, algorithms/sssp/sssp_blast.hpp:8-72,
-// for k sources): the Bellman-Ford fixed point computed over the set of
-// vertices improved in the last round; a lane group of k lanes relaxes one
-// edge for all k sources, int32 distances saturate at INT32_MAX.
+// for k sources): dense Bellman-Ford rounds, each one min.plus SpMM over the
+// k distance columns (int32, saturating at INT32_MAX), until no column
+// changes: the reference's own formulation (a dense vxm with MinimumPlus per
+// iteration), batched over sources.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -260,123 +261,57 @@ int ppr_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_by
 }
 
 // ------------------------------------------------------------ multi-SSSP --
-template <int LW>
-__global__ __launch_bounds__(256) void k_mssp_relax(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                    const int32_t* __restrict__ w, const int32_t* __restrict__ q,
-                                                    int64_t nq, const int32_t* __restrict__ offs, int64_t nedges,
-                                                    int K, int32_t* __restrict__ D, int32_t* __restrict__ inq,
-                                                    int32_t* __restrict__ nq_out, int32_t* __restrict__ nq_count,
-                                                    int32_t round, unsigned long long* __restrict__ changed) {
-    constexpr int GPB = 256 / LW;
-    constexpr int EPG = 4;  // edges per group
-    const int64_t g = (int64_t)blockIdx.x * GPB + threadIdx.x / LW;
-    const int lane = threadIdx.x % LW;
-    const int64_t e0 = g * EPG;
-    bool take_any = false;
-    int32_t vtake = 0;
-    unsigned long long chg = 0;
-    if (e0 < nedges) {
-        int64_t lo = 0, hi = nq - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (offs[mid] <= e0) lo = mid;
-            else hi = mid - 1;
-        }
-        int64_t i = lo;
-        const int64_t e1 = min(e0 + EPG, nedges);
-        for (int64_t e = e0; e < e1; ++e) {
-            while (i + 1 < nq && offs[i + 1] <= e) ++i;
-            const int u = q[i];
-            const int64_t kk = rp[u] + (e - offs[i]);
-            const int v = col[kk];
-            const int32_t wt = w[kk];
-            bool imp = false;
-            if (lane < K) {
-                const int32_t du = D[(int64_t)u * K + lane];
-                if (du != INT32_MAX) {
-                    const int32_t nd = sat_add<int32_t>(du, wt);
-                    int32_t* pv = D + (int64_t)v * K + lane;
-                    if (nd < *pv && atomicMin(pv, nd) > nd) imp = true;
-                }
-            }
-            const unsigned long long b = __ballot(imp);
-            // bits of this group only
-            const int gbase = (threadIdx.x & 63) & ~(LW - 1);
-            const unsigned long long gm = (LW == 64) ? ~0ull : (((1ull << LW) - 1) << gbase);
-            const unsigned long long mine = b & gm;
-            if (mine) {
-                chg |= (mine >> gbase);
-                if (lane == 0 && atomicExch(inq + v, round) != round) {
-                    take_any = true;
-                    vtake = v;
-                }
-            }
-            // one enqueue slot per thread: flush it before the next edge
-            {
-                const unsigned long long tb = __ballot(take_any);
-                if (tb) {
-                    const int wl = threadIdx.x & 63;
-                    const int leader = __ffsll((long long)tb) - 1;
-                    int basei = 0;
-                    if (wl == leader) basei = atomicAdd(nq_count, (int)__popcll(tb));
-                    basei = __shfl(basei, leader, 64);
-                    if (take_any) {
-                        const unsigned long long lt = wl == 0 ? 0ull : (~0ull >> (64 - wl));
-                        nq_out[basei + __popcll(tb & lt)] = vtake;
-                    }
-                }
-                take_any = false;
-            }
-        }
-    }
-    // columns that improved somewhere this round (bit q = source q)
-    if (lane == 0 && chg) atomicOr(changed, chg);
-}
+// Dense Bellman-Ford rounds: each round is one min.plus SpMM over the k
+// distance columns (Y = A (min.+) D, saturating int32) and an element-wise
+// D = min(D, Y) that records which columns changed.  A column that did not
+// change in a round is at its fixed point, the shortest distances.  The
+// rounds run on the merge-path SpMM, with no atomics on distances.
 
-__global__ void k_mssp_init(int32_t* __restrict__ D, int64_t n, int K, int32_t* __restrict__ inq) {
+// D[v][q] = INT32_MAX for every q in `cols` (a bit mask), and 0 at the
+// column's source.
+__global__ void k_sssp_reset_cols(int32_t* __restrict__ D, int64_t n, int k, uint64_t cols,
+                                  const int32_t* __restrict__ src) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n * K) D[i] = INT32_MAX;
-    if (i < n) inq[i] = -1;
+    if (i >= n * k) return;
+    const int q = (int)(i % k);
+    if (!((cols >> q) & 1ull)) return;
+    const int64_t v = i / k;
+    D[i] = (src[q] == v) ? 0 : INT32_MAX;
 }
 
-__global__ void k_mssp_sources(int32_t* __restrict__ D, int K, const int32_t* __restrict__ src,
-                               int32_t* __restrict__ q, int32_t* __restrict__ qn, int32_t* __restrict__ inq) {
-    if (threadIdx.x != 0) return;
-    int cnt = 0;
-    for (int s = 0; s < K; ++s) {
-        const int v = src[s];
-        D[(int64_t)v * K + s] = 0;
-        if (inq[v] != -2) {
-            inq[v] = -2;
-            q[cnt++] = v;
+// D = min(D, Y); bit q of *changed is set if column q improved anywhere.
+__global__ __launch_bounds__(256) void k_sssp_min_update(int32_t* __restrict__ D, const int32_t* __restrict__ Y,
+                                                         int64_t total, int k, unsigned long long* __restrict__ changed) {
+    __shared__ unsigned long long s_m;
+    if (threadIdx.x == 0) s_m = 0;
+    __syncthreads();
+    unsigned long long m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t y = Y[i];
+        if (y < D[i]) {
+            D[i] = y;
+            m |= 1ull << (int)(i % k);
         }
     }
-    for (int s = 0; s < K; ++s) inq[src[s]] = -1;
-    *qn = cnt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off, 64);
+    if ((threadIdx.x & 63) == 0 && m) atomicOr(&s_m, m);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_m) atomicOr(changed, s_m);
 }
 
-__global__ void k_zero_i32(int32_t* __restrict__ p, int n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0;
-}
-
-__global__ void k_queue_deg(const int32_t* __restrict__ rp, const int32_t* __restrict__ q, int64_t nq,
-                            int32_t* __restrict__ deg) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nq) {
-        const int u = q[i];
-        deg[i] = rp[u + 1] - rp[u];
-    }
+__global__ void k_zero_mask(unsigned long long* m) {
+    if (threadIdx.x == 0) *m = 0;
 }
 
 struct MsspLayout {
-    int32_t *D, *inq, *q0, *q1, *deg, *offs, *cnt, *src;
-    void* partial;
-    size_t part_bytes;
+    int32_t *D, *Y, *src;
+    void* carry;
     unsigned long long* changed;
     size_t total;
 };
 
-static MsspLayout mssp_layout(int64_t n, int k, char* base) {
+static MsspLayout mssp_layout(const DevCSR& m, int64_t n, int k, char* base) {
     MsspLayout L;
     size_t off = 0;
     auto take = [&](size_t bytes) -> char* {
@@ -385,22 +320,36 @@ static MsspLayout mssp_layout(int64_t n, int k, char* base) {
         return r;
     };
     L.D = (int32_t*)take(sizeof(int32_t) * (size_t)n * k);
-    // queues hold at most n distinct vertices plus up to 64 seeds
-    L.inq = (int32_t*)take(sizeof(int32_t) * (n + 1));
-    L.q0 = (int32_t*)take(sizeof(int32_t) * (n + 66));
-    L.q1 = (int32_t*)take(sizeof(int32_t) * (n + 66));
-    L.deg = (int32_t*)take(sizeof(int32_t) * (n + 66));
-    L.offs = (int32_t*)take(sizeof(int32_t) * (n + 66));
-    L.part_bytes = scan_partial_bytes(n + 66);
-    L.partial = take(L.part_bytes);
-    L.cnt = (int32_t*)take(sizeof(int32_t) * 4);
+    L.Y = (int32_t*)take(sizeof(int32_t) * (size_t)n * k);
+    L.carry = take(spmm_carry_bytes(m, GBK_INT32, k));
     L.src = (int32_t*)take(sizeof(int32_t) * 64);
     L.changed = (unsigned long long*)take(sizeof(unsigned long long) * 2);
     L.total = off;
     return L;
 }
 
-size_t mssp_ws_bytes(int64_t n, int k) { return mssp_layout(n, k, nullptr).total + 256; }
+size_t mssp_ws_bytes(gbk_matrix A, int k) { return mssp_layout(A->csr, A->n, k, nullptr).total + 256; }
+
+static dim3 capped_b(int64_t threads) {
+    int64_t b = (threads + 255) / 256;
+    if (b > 8192) b = 8192;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+// One round: Y = A (min.+) D, D = min(D, Y); *changed_out receives the mask of
+// columns that changed.
+static int mssp_round(gbk_matrix A, const MsspLayout& L, int k, hipStream_t st, unsigned long long* hmask,
+                      uint64_t* changed_out) {
+    const int64_t n = A->n;
+    GBK_TRY(spmm_run(A->csr, GBK_INT32, L.D, L.Y, k, GBK_ADD_MIN, GBK_MUL_PLUS, GBK_ACC_NONE, L.carry, st));
+    hipLaunchKernelGGL(k_zero_mask, dim3(1), dim3(64), 0, st, L.changed);
+    hipLaunchKernelGGL(k_sssp_min_update, capped_b(n * k), dim3(256), 0, st, L.D, L.Y, n * k, k, L.changed);
+    GBK_LAUNCH_CHECK();
+    GBK_HIP(hipMemcpyAsync(hmask, L.changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    *changed_out = *hmask;
+    return GBK_SUCCESS;
+}
 
 // dist_out: k*n int32, source-major, on the device.
 int mssp_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_bytes, int32_t* dist_out,
@@ -408,64 +357,29 @@ int mssp_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_b
     const int64_t n = A->n;
     if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "multi-source SSSP takes 1..64 sources (got %d)", k);
     if (A->type != GBK_INT32) return fail(GBK_DOMAIN_MISMATCH, "multi-source SSSP needs an INT32 weight matrix");
-    if (ws_bytes < mssp_ws_bytes(n, k)) return fail(GBK_INSUFFICIENT_SPACE, "SSSP workspace too small");
-    const DevCSR& m = A->csr;
+    if (ws_bytes < mssp_ws_bytes(A, k)) return fail(GBK_INSUFFICIENT_SPACE, "SSSP workspace too small");
+    GBK_TRY(ensure_spmm_tiles(A->csr, st));
     char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
-    MsspLayout L = mssp_layout(n, k, base);
+    MsspLayout L = mssp_layout(A->csr, n, k, base);
     GBK_HIP(hipMemcpyAsync(L.src, src_host, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_mssp_init, grid_b(std::max<int64_t>(n * k, n)), dim3(256), 0, st, L.D, n, k, L.inq);
+    const uint64_t all = k == 64 ? ~0ull : ((1ull << k) - 1);
+    hipLaunchKernelGGL(k_sssp_reset_cols, grid_b(n * k), dim3(256), 0, st, L.D, n, k, all, L.src);
     GBK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L.cnt, 4);
-    GBK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mssp_sources, dim3(1), dim3(64), 0, st, L.D, k, L.src, L.q0, L.cnt, L.inq);
-    GBK_LAUNCH_CHECK();
-    int32_t nq32 = 0;
-    GBK_HIP(hipMemcpyAsync(&nq32, L.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    GBK_HIP(hipStreamSynchronize(st));
-    int64_t nq = nq32, rounds = 0, relaxed = 0;
-    int32_t* qa = L.q0;
-    int32_t* qb = L.q1;
-    const int LW = k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
-    while (nq > 0) {
-        hipLaunchKernelGGL(k_queue_deg, grid_b(nq), dim3(256), 0, st, m.row_ptr, qa, nq, L.deg);
-        GBK_LAUNCH_CHECK();
-        int32_t ne = 0;
-        GBK_TRY(exclusive_scan_i32_on(L.deg, L.offs, nq, &ne, st, L.partial, L.part_bytes));
-        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L.cnt, 4);
-        GBK_LAUNCH_CHECK();
-        if (ne > 0) {
-            const int64_t groups = ((int64_t)ne + 3) / 4;
-#define GBK_MSSP(LWV)                                                                                          \
-    case LWV:                                                                                                  \
-        hipLaunchKernelGGL(k_mssp_relax<LWV>, grid_b(groups, 256 / LWV), dim3(256), 0, st, m.row_ptr, m.col,    \
-                           (const int32_t*)m.val, qa, nq, L.offs, (int64_t)ne, k, L.D, L.inq, qb, L.cnt,         \
-                           (int32_t)rounds, L.changed);                                                        \
-        break;
-            switch (LW) {
-                GBK_MSSP(8)
-                GBK_MSSP(16)
-                GBK_MSSP(32)
-                GBK_MSSP(64)
-            }
-#undef GBK_MSSP
-            GBK_LAUNCH_CHECK();
-        }
-        relaxed += ne;
-        GBK_HIP(hipMemcpyAsync(&nq32, L.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        GBK_HIP(hipStreamSynchronize(st));
-        nq = nq32;
-        std::swap(qa, qb);
+    unsigned long long hmask = 0;
+    int64_t rounds = 0;
+    uint64_t changed = all;
+    while (changed) {
+        GBK_TRY(mssp_round(A, L, k, st, &hmask, &changed));
         ++rounds;
     }
     hipLaunchKernelGGL(k_transpose_nk<int32_t>, grid_b(n, 64), dim3(256), 0, st, L.D, n, k, dist_out);
     GBK_LAUNCH_CHECK();
     if (stats) {
         stats[0] = rounds;
-        stats[1] = relaxed;
+        stats[1] = rounds * A->csr.nnz;
     }
     return GBK_SUCCESS;
 }
-
 
 // --------------------------------------------------- PPR / SSSP engines --
 // Column q of the dense block belongs to slot q.  Seeding a slot rewrites its
@@ -485,15 +399,6 @@ template <typename T>
 __global__ void k_extract_col(const T* __restrict__ M, int64_t n, int k, int q, T* __restrict__ out) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n) out[v] = M[v * k + q];
-}
-
-// D[v][slot[i]] = (v == src[i]) ? 0 : INT32_MAX; seeds join the queue at qa[nq + i].
-__global__ void k_sssp_seed(int32_t* __restrict__ D, int64_t n, int k, const int32_t* __restrict__ src,
-                            const int32_t* __restrict__ slot, int count, int32_t* __restrict__ qa, int64_t nq) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n)
-        for (int i = 0; i < count; ++i) D[v * k + slot[i]] = (src[i] == v) ? 0 : INT32_MAX;
-    if (v < count) qa[nq + v] = src[v];
 }
 
 namespace {
@@ -634,23 +539,23 @@ class SsspEngine : public Engine {
     SsspEngine(gbk_matrix A, hipStream_t st, int k) : A_(A), st_(st), n_(A->n), k_(k) {}
     ~SsspEngine() override {
         dev_free(ws_);
-        dev_free(seeds_);
-        if (hbuf_) (void)hipHostFree(hbuf_);
+        if (hmask_) (void)hipHostFree(hmask_);
     }
     int init() {
         if (A_->type != GBK_INT32) return fail(GBK_DOMAIN_MISMATCH, "SSSP needs an INT32 weight matrix");
-        ws_bytes_ = mssp_ws_bytes(n_, k_) + 512;
+        GBK_TRY(ensure_spmm_tiles(A_->csr, st_));
+        ws_bytes_ = mssp_ws_bytes(A_, k_) + 512;
         GBK_TRY(dev_alloc(&ws_, ws_bytes_));
-        GBK_TRY(dev_alloc((void**)&seeds_, sizeof(int32_t) * 128));
-        GBK_HIP(hipHostMalloc((void**)&hbuf_, 64, hipHostMallocDefault));
+        GBK_HIP(hipHostMalloc((void**)&hmask_, sizeof(unsigned long long), hipHostMallocDefault));
         char* base = (char*)(((uintptr_t)ws_ + 255) & ~(uintptr_t)255);
-        L_ = mssp_layout(n_, k_, base);
-        hipLaunchKernelGGL(k_mssp_init, grid_b(std::max<int64_t>(n_ * k_, n_)), dim3(256), 0, st_, L_.D, n_, k_, L_.inq);
+        L_ = mssp_layout(A_->csr, n_, k_, base);
+        // every column starts as "source 0 at distance 0", unused until seeded
+        std::vector<int32_t> zeros(64, 0);
+        GBK_HIP(hipMemcpyAsync(L_.src, zeros.data(), sizeof(int32_t) * 64, hipMemcpyHostToDevice, st_));
+        const uint64_t all = k_ == 64 ? ~0ull : ((1ull << k_) - 1);
+        hipLaunchKernelGGL(k_sssp_reset_cols, grid_b(n_ * k_), dim3(256), 0, st_, L_.D, n_, k_, all, L_.src);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipStreamSynchronize(st_));
-        qa_ = L_.q0;
-        qb_ = L_.q1;
-        LW_ = k_ <= 8 ? 8 : k_ <= 16 ? 16 : k_ <= 32 ? 32 : 64;
         return GBK_SUCCESS;
     }
     int slots() const override { return k_; }
@@ -664,61 +569,23 @@ class SsspEngine : public Engine {
     }
     int step(std::vector<int>& finished) override {
         finished.clear();
-        const DevCSR& m = A_->csr;
-        uint64_t seeded = 0;
         if (!pend_slot_.empty()) {
-            std::vector<int32_t> buf(pend_src_);
-            buf.insert(buf.end(), pend_slot_.begin(), pend_slot_.end());
-            GBK_HIP(hipMemcpyAsync(seeds_, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, st_));
-            const int cnt = (int)pend_slot_.size();
-            hipLaunchKernelGGL(k_sssp_seed, grid_b(std::max<int64_t>(n_, cnt)), dim3(256), 0, st_, L_.D, n_, k_, seeds_,
-                               seeds_ + cnt, cnt, qa_, nq_);
-            GBK_LAUNCH_CHECK();
-            nq_ += cnt;
-            for (int q : pend_slot_) {
-                occupied_ |= 1ull << q;
-                seeded |= 1ull << q;
+            uint64_t cols = 0;
+            for (size_t i = 0; i < pend_slot_.size(); ++i) {
+                src_[pend_slot_[i]] = pend_src_[i];
+                cols |= 1ull << pend_slot_[i];
             }
-            GBK_HIP(hipStreamSynchronize(st_));
+            GBK_HIP(hipMemcpyAsync(L_.src, src_, sizeof(int32_t) * k_, hipMemcpyHostToDevice, st_));
+            hipLaunchKernelGGL(k_sssp_reset_cols, grid_b(n_ * k_), dim3(256), 0, st_, L_.D, n_, k_, cols, L_.src);
+            GBK_LAUNCH_CHECK();
+            occupied_ |= cols;
+            GBK_HIP(hipStreamSynchronize(st_));  // src_ may change before the next copy
             pend_src_.clear();
             pend_slot_.clear();
         }
         if (!occupied_) return GBK_SUCCESS;
         uint64_t changed = 0;
-        if (nq_ > 0) {
-            hipLaunchKernelGGL(k_queue_deg, grid_b(nq_), dim3(256), 0, st_, m.row_ptr, qa_, nq_, L_.deg);
-            GBK_LAUNCH_CHECK();
-            int32_t ne = 0;
-            GBK_TRY(exclusive_scan_i32_on(L_.deg, L_.offs, nq_, &ne, st_, L_.partial, L_.part_bytes));
-            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st_, L_.cnt, 4);
-            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st_, (int32_t*)L_.changed, 4);
-            GBK_LAUNCH_CHECK();
-            if (ne > 0) {
-                const int64_t groups = ((int64_t)ne + 3) / 4;
-#define GBK_SMSSP(LWV)                                                                                        \
-    case LWV:                                                                                                 \
-        hipLaunchKernelGGL(k_mssp_relax<LWV>, grid_b(groups, 256 / LWV), dim3(256), 0, st_, m.row_ptr, m.col,  \
-                           (const int32_t*)m.val, qa_, nq_, L_.offs, (int64_t)ne, k_, L_.D, L_.inq, qb_, L_.cnt,  \
-                           round_, L_.changed);                                                               \
-        break;
-                switch (LW_) {
-                    GBK_SMSSP(8)
-                    GBK_SMSSP(16)
-                    GBK_SMSSP(32)
-                    GBK_SMSSP(64)
-                }
-#undef GBK_SMSSP
-                GBK_LAUNCH_CHECK();
-            }
-            GBK_HIP(hipMemcpyAsync(hbuf_, L_.cnt, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
-            GBK_HIP(hipMemcpyAsync(hbuf_ + 2, L_.changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
-            GBK_HIP(hipStreamSynchronize(st_));
-            nq_ = hbuf_[0];
-            changed = *(unsigned long long*)(hbuf_ + 2);
-            std::swap(qa_, qb_);
-            ++round_;
-        }
-        (void)seeded;
+        GBK_TRY(mssp_round(A_, L_, k_, st_, hmask_, &changed));
         for (int q = 0; q < k_; ++q)
             if (((occupied_ >> q) & 1ull) && !((changed >> q) & 1ull)) finished.push_back(q);
         return GBK_SUCCESS;
@@ -736,15 +603,11 @@ class SsspEngine : public Engine {
     int64_t n_;
     int k_;
     void* ws_ = nullptr;
-    int32_t* seeds_ = nullptr;
-    int32_t* hbuf_ = nullptr;
+    unsigned long long* hmask_ = nullptr;
     size_t ws_bytes_ = 0;
     MsspLayout L_{};
-    int32_t *qa_ = nullptr, *qb_ = nullptr;
-    int64_t nq_ = 0;
-    int32_t round_ = 0;
-    int LW_ = 32;
     uint64_t occupied_ = 0;
+    int32_t src_[64] = {0};
     std::vector<int32_t> pend_src_, pend_slot_;
 };
 
@@ -813,7 +676,7 @@ extern "C" int gbk_sssp_multi(gbk_matrix A, const int64_t* sources, int k, int32
     GBK_TRY(ensure_init());
     hipStream_t st = stream();
     void *ws, *out;
-    const size_t wb = mssp_ws_bytes(A->n, k);
+    const size_t wb = mssp_ws_bytes(A, k);
     GBK_TRY(scratch(SLOT_MSSSP, wb, &ws));
     GBK_TRY(scratch(SLOT_MSBFS_OUT, sizeof(int32_t) * (size_t)k * A->n, &out));
     GBK_TRY(mssp_run(A, s32.data(), k, ws, wb, (int32_t*)out, stats, st));

@@ -134,8 +134,9 @@ int64_t engine_estimate(gbk_matrix A, int kind, int k) {
     // workspace estimate used for admission before the engine exists
     const int64_t n = A->n;
     if (kind == GBK_QUERY_BFS) return (int64_t)msbfs_ws_bytes(n) + 1024;
-    if (kind == GBK_QUERY_PPR) return 2 * 4 * n * k + 2 * 4 * k * ((n + A->nnz) / SPMM_TILE + 1) + 16 * (n + 1) + 8192;
-    return 4 * n * k + 24 * (n + 66) + 8192;
+    // the engines' own workspace sizes, plus the seed and host buffers
+    if (kind == GBK_QUERY_PPR) return (int64_t)ppr_ws_bytes(A, k) + 512 + 8192;
+    return (int64_t)mssp_ws_bytes(A, k) + 512 + 8192;
 }
 
 // One worker: the queries of one (graph, kind), in arrival order.
