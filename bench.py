@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed loop, copy this many MiB device-to-device on the library stream "
                          "(a kernel of known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal"],
+                    help="run one secondary measurement alone (for profiling); prints that section as JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -70,6 +72,17 @@ def main():
 
     import gb_kun_amd as g
     from gb_kun_amd.device import DeviceArray, Event
+
+    if args.section != "all":
+        if args.section == "batched":
+            rp, col = g.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1, symmetric=False, permute=True)
+            res = batched_bench(g, rp, col, len(rp) - 1, args.scale)
+        elif args.section == "served":
+            res = served_bench(g, full=args.serve_full)
+        else:
+            res = traversal_bench(g, args.bfs_scale)
+        print(json.dumps({"section": args.section, args.section: res}), flush=True)
+        return
 
     t0 = time.time()
     if args.graph == "rmat":
@@ -304,12 +317,16 @@ def traversal_bench(g, scale, nsrc=64, seed=13):
     As = g.Matrix.from_csr(rp, col, w)
     log(f"traversal graph: symmetric R-MAT {scale}: n={n} stored nnz={len(col)} ({time.perf_counter() - t0:.1f} s)")
     src = _sources(rp, nsrc, seed + 2)
-    g.bfs(Ab, int(src[0]))
+    from gb_kun_amd.device import DeviceArray, synchronize
+    out = DeviceArray(n, np.int32)  # levels / distances stay on the device while timed
+    g.bfs_device(Ab, int(src[0]), out.ptr)
     teps, times, levels, dirs = [], [], [], [0, 0]
     for s_ in src:
         t1 = time.perf_counter()
-        lv, st = g.bfs(Ab, int(s_))
+        st = g.bfs_device(Ab, int(s_), out.ptr)
+        synchronize()
         dt = time.perf_counter() - t1
+        lv = out.numpy()
         m = float(deg[lv > 0].sum()) / 2.0
         teps.append(m / dt)
         times.append(dt)
@@ -318,16 +335,19 @@ def traversal_bench(g, scale, nsrc=64, seed=13):
         dirs[1] += int(st[2])
     bfs = {"gteps_hmean": round(len(teps) / sum(1.0 / t for t in teps) / 1e9, 3),
            "ms_median": round(float(np.median(times)) * 1e3, 3), "levels_median": int(np.median(levels)),
-           "push_steps": dirs[0], "pull_steps": dirs[1]}
-    g.sssp(As, int(src[0]))
+           "push_steps": dirs[0], "pull_steps": dirs[1], "sources": len(src)}
+    g.sssp_device(As, int(src[0]), out.ptr)
     teps, times = [], []
     for s_ in src[:16]:
         t1 = time.perf_counter()
-        d, st = g.sssp(As, int(s_))
+        g.sssp_device(As, int(s_), out.ptr)
+        synchronize()
         dt = time.perf_counter() - t1
+        d = out.numpy()
         m = float(deg[d != np.iinfo(np.int32).max].sum()) / 2.0
         teps.append(m / dt)
         times.append(dt)
+    out.free()
     sssp = {"gteps_hmean": round(len(teps) / sum(1.0 / t for t in teps) / 1e9, 3),
             "ms_median": round(float(np.median(times)) * 1e3, 3), "sources": 16}
     log(f"BFS symmetric R-MAT {scale}: {bfs['gteps_hmean']} GTEPS (median {bfs['ms_median']} ms); "

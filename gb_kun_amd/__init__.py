@@ -128,6 +128,7 @@ _sig("gbk_reduce_i64", C.c_int, _pi64, C.c_int, C.c_int, _vp, C.POINTER(Desc))
 _sig("gbk_extract", C.c_int, _vp, _vp, C.c_int, _vp, _vp, C.POINTER(Desc))
 _sig("gbk_bfs", C.c_int, _vp, _i64, _pi32, _pi64)
 _sig("gbk_bfs_device", C.c_int, _vp, _i64, _vp, _pi64)
+_sig("gbk_sssp_device", C.c_int, _vp, _i64, _vp, _pi64)
 _sig("gbk_sssp", C.c_int, _vp, _i64, _vp, _pi64)
 _sig("gbk_cc", C.c_int, _vp, _pi32, _pi64)
 _sig("gbk_pagerank", C.c_int, _vp, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
@@ -448,6 +449,13 @@ def sssp(A, source):
     st = np.zeros(8, dtype=np.int64)
     check(lib.gbk_sssp(A.h, source, _ptr(d), _ptr(st, C.c_int64)))
     return d, st[:2]
+
+
+def sssp_device(A, source, dist_ptr):
+    """SSSP writing n distances of the matrix type to a device array; returns the stats."""
+    st = np.zeros(8, dtype=np.int64)
+    check(lib.gbk_sssp_device(A.h, source, _vp(dist_ptr), _ptr(st, C.c_int64)))
+    return st[:2]
 
 
 def cc(A):

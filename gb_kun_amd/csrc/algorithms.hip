@@ -662,16 +662,24 @@ extern "C" int gbk_bfs(gbk_matrix A, int64_t source, int32_t* levels, int64_t* s
     return GBK_SUCCESS;
 }
 
-extern "C" int gbk_sssp(gbk_matrix A, int64_t source, void* dist, int64_t* stats) {
+static int sssp_into(gbk_matrix A, int64_t source, void* d, int64_t* stats) {
     GBK_TRY(check_graph(A));
-    if (!dist) return fail(GBK_NULL_POINTER, "null distances");
+    if (!d) return fail(GBK_NULL_POINTER, "null distances");
     if (source < 0 || source >= A->n) return fail(GBK_INVALID_INDEX, "source %lld out of range", (long long)source);
+    if (A->type == GBK_INT32) return sssp_run<int32_t>(A, source, (int32_t*)d, stats);
+    if (A->type == GBK_FP32) return sssp_run<float>(A, source, (float*)d, stats);
+    return fail(GBK_DOMAIN_MISMATCH, "SSSP needs INT32 or FP32 weights");
+}
+
+extern "C" int gbk_sssp_device(gbk_matrix A, int64_t source, void* dist_dev, int64_t* stats) {
+    return sssp_into(A, source, dist_dev, stats);
+}
+
+extern "C" int gbk_sssp(gbk_matrix A, int64_t source, void* dist, int64_t* stats) {
+    if (!dist) return fail(GBK_NULL_POINTER, "null distances");
+    GBK_TRY(check_graph(A));
     void* d = A->ws.buf[3];
-    int rc;
-    if (A->type == GBK_INT32) rc = sssp_run<int32_t>(A, source, (int32_t*)d, stats);
-    else if (A->type == GBK_FP32) rc = sssp_run<float>(A, source, (float*)d, stats);
-    else return fail(GBK_DOMAIN_MISMATCH, "SSSP needs INT32 or FP32 weights");
-    GBK_TRY(rc);
+    GBK_TRY(sssp_into(A, source, d, stats));
     GBK_HIP(hipMemcpyAsync(dist, d, 4 * A->n, hipMemcpyDeviceToHost, stream()));
     GBK_HIP(hipStreamSynchronize(stream()));
     return GBK_SUCCESS;

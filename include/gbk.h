@@ -341,6 +341,8 @@ int gbk_bfs_device(gbk_matrix A, int64_t source, int32_t* levels_dev, int64_t* s
  * unreachable = INT32_MAX; FP32 weights: distances float, unreachable =
  * FLT_MAX.  dist is a host array of the matrix type. */
 int gbk_sssp(gbk_matrix A, int64_t source, void* dist, int64_t* stats);
+/* Same, writing the distances (n entries of the matrix type) to a device array. */
+int gbk_sssp_device(gbk_matrix A, int64_t source, void* dist_dev, int64_t* stats);
 
 /* Connected components, replaces lablas::algorithm::cc (FastSV,
  * algorithms/cc/cc.hpp:12-119).  labels (host int32): the minimum vertex id

@@ -183,6 +183,24 @@ def test_sssp_vs_dijkstra(gbk, oracle, scale):
         assert np.array_equal(got, oracle.dijkstra_i32(rp, col, w, int(s)))
 
 
+def test_sssp_and_bfs_device_outputs(gbk, oracle):
+    from gb_kun_amd.device import DeviceArray
+    rp, col = gbk.gen_rmat(13, 16, seed=91, symmetric=True)
+    n = len(rp) - 1
+    w = gbk.sym_weights_i32(rp, col, 92, 1, 255)
+    A = gbk.Matrix.from_csr(rp, col, w)
+    Ab = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    buf = DeviceArray.zeros(n, np.int32)
+    try:
+        for s in _sources(rp, 2, 93):
+            gbk.sssp_device(A, int(s), buf.ptr)
+            assert np.array_equal(buf.numpy(), oracle.dijkstra_i32(rp, col, w, int(s)))
+            gbk.bfs_device(Ab, int(s), buf.ptr)
+            assert np.array_equal(buf.numpy(), gbk.bfs(Ab, int(s))[0])
+    finally:
+        buf.free()
+
+
 # --------------------------------------------------------------------- CC
 def test_cc_golden(gbk):
     d = load("cc_two_blocks")
