@@ -17,6 +17,7 @@
 // pair sums in segment order (deterministic, no atomics).  LAV first gathers
 // x into the degree-sorted column order.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "gbk_objects.h"
@@ -44,6 +45,21 @@ struct SellC {
     DevCSR lng;                       // CSR of the long rows (row i = long_rows[i])
 };
 
+// Hot columns of a segment in LDS: the persistent kernel's blocks on XCD x
+// serve the segments s with s % 8 == x; each block first copies the hot x
+// values of its segment into LDS, so entries of hot columns read LDS and the
+// rest read the segment's x slice through that XCD's L2.
+struct SegHot {
+    int H = 0;                    // LDS slots per segment
+    int64_t ntiles = 0;           // wave tiles (HOT_WT merge items each)
+    int32_t* hcol = nullptr;      // entries, hot columns as -1 - slot
+    int32_t* hot_ids = nullptr;   // nseg * H original columns (-1: unused slot)
+    void* xh = nullptr;           // nseg * H gathered x values, per call
+    int2* tiles = nullptr;        // ntiles + 1
+    int64_t* seg_tile = nullptr;  // nseg + 1
+    void* carry = nullptr;        // 2 * ntiles values
+};
+
 struct SegCSR {
     int nseg = 0;
     int64_t npairs = 0;
@@ -53,6 +69,7 @@ struct SegCSR {
     int32_t* row_pairs = nullptr;
     void* partial = nullptr;
     int64_t n = 0;
+    SegHot hot;  // built for CSR_SEG (hot.tiles == nullptr otherwise)
 };
 
 struct Lav {
@@ -241,6 +258,12 @@ static void free_seg_fields(SegCSR& s) {
     dev_free(s.rpp);
     dev_free(s.row_pairs);
     dev_free(s.partial);
+    dev_free(s.hot.hcol);
+    dev_free(s.hot.hot_ids);
+    dev_free(s.hot.xh);
+    dev_free(s.hot.tiles);
+    dev_free(s.hot.seg_tile);
+    dev_free(s.hot.carry);
     s = SegCSR();
 }
 
@@ -303,20 +326,58 @@ static int64_t seg_cols_default(int64_t n) {
     return 512 * 1024;
 }
 
+// Persistent hot-column kernel shape: 16 waves per block, each wave merges
+// tiles of HOT_WT items (8 per lane) on its own LDS staging area.
+constexpr int HOT_WAVES = 16;
+constexpr int HOT_WT = 512;
+
+static size_t hot_staging_bytes(size_t vs) { return (size_t)HOT_WAVES * ((HOT_WT + 1) * 4 + HOT_WT * vs); }
+
+// LDS slots per segment for values of vs bytes: what one workgroup's LDS
+// holds beside the staging areas, a multiple of 64.
+static int hot_slots(size_t vs) {
+    const int64_t lds = device_lds_bytes();
+    const int64_t room = lds - (int64_t)hot_staging_bytes(vs);
+    if (room <= 0) return 0;
+    int64_t h = (room / (int64_t)vs) & ~(int64_t)63;
+    return h < 1024 ? 0 : (int)std::min<int64_t>(h, 1 << 20);
+}
+
+static int build_seg_hot_dev(const gbk_host::SegHost& h, int type, int64_t n, SegCSR& s) {
+    const size_t vs = type_size(type);
+    gbk_host::SegHotHost hh;
+    gbk_host::build_seg_hot(h, n, hot_slots(vs), HOT_WT, hh);
+    SegHot& o = s.hot;
+    o.H = hh.H;
+    o.ntiles = (int64_t)hh.tiles.size() / 2 - 1;
+    GBK_TRY(upload(&o.hcol, hh.hcol));
+    GBK_TRY(upload(&o.hot_ids, hh.hot_ids));
+    GBK_TRY(dev_alloc(&o.xh, vs * (hh.hot_ids.empty() ? 1 : hh.hot_ids.size())));
+    std::vector<int2> tiles(o.ntiles + 1);
+    for (int64_t t = 0; t <= o.ntiles; ++t) tiles[t] = make_int2(hh.tiles[2 * t], hh.tiles[2 * t + 1]);
+    GBK_TRY(upload(&o.tiles, tiles));
+    GBK_TRY(upload(&o.seg_tile, hh.seg_tile));
+    GBK_TRY(dev_alloc(&o.carry, 2 * vs * (size_t)(o.ntiles > 0 ? o.ntiles : 1)));
+    return GBK_SUCCESS;
+}
+
 static int build_seg(const DevCSR& m, int type, SegCSR** out) {
     std::vector<int32_t> rp, col;
     std::vector<char> val;
     GBK_TRY(download_csr(m, type, rp, col, val));
     const int64_t n = m.nrows;
-    std::vector<int64_t> bounds;
+    // a multiple of 8 segments (one group per XCD), each at most 512 Ki
+    // columns wide
     const int64_t sc = seg_cols_default(n);
-    for (int64_t b = 0; b < n; b += sc) bounds.push_back(b);
-    bounds.push_back(n);
-    if (bounds.size() < 2) bounds = {0, n};
+    const int64_t nseg = 8 * std::max<int64_t>(1, (n + 8 * sc - 1) / (8 * sc));
+    const int64_t w = std::max<int64_t>(1, (n + nseg - 1) / nseg);
+    std::vector<int64_t> bounds;
+    for (int64_t i = 0; i <= nseg; ++i) bounds.push_back(std::min(n, i * w));
     gbk_host::SegHost h;
     gbk_host::build_segmented(n, rp.data(), col.data(), val.data(), type_size(type), bounds, nullptr, h);
     SegCSR* s = new SegCSR();
     int rc = build_seg_common(h, type, n, *s);
+    if (rc == GBK_SUCCESS) rc = build_seg_hot_dev(h, type, n, *s);
     if (rc != GBK_SUCCESS) {
         free_seg_fields(*s);
         delete s;
@@ -336,7 +397,212 @@ __global__ void k_seg_merge(const int32_t* __restrict__ rpp, const int32_t* __re
     y[r] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[r], acc);
 }
 
+template <typename T> __device__ __forceinline__ T hot_shfl_up(T v, int off) {
+    if constexpr (sizeof(T) >= 4) return __shfl_up(v, off, 64);
+    else return (T)__shfl_up((int)v, off, 64);
+}
+template <typename T> __device__ __forceinline__ T hot_shfl(T v, int lane) {
+    if constexpr (sizeof(T) >= 4) return __shfl(v, lane, 64);
+    else return (T)__shfl((int)v, lane, 64);
+}
+
+// xh[s * H + i] = x[hot_ids[s * H + i]] (unused slots: the identity, never read).
+template <typename T, int ADD>
+__global__ void k_hot_x(const int32_t* __restrict__ ids, const T* __restrict__ x, T* __restrict__ xh, int64_t m) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        const int32_t c = ids[i];
+        xh[i] = c >= 0 ? x[c] : add_identity<ADD, T>();
+    }
+}
+
+// Persistent merge-path SpMV over the (segment, row) pairs with the hot x of
+// the block's segment in LDS.  Grid: a multiple of 8 workgroups; workgroup b
+// runs on XCD b % 8 and serves segments b % 8, b % 8 + 8, ...; within a
+// segment, wave w of block j takes tiles seg_tile[s] + j * HOT_WAVES + w in
+// steps of (grid / 8) * HOT_WAVES.  Each wave merges its 512-item tile alone:
+// row ends and products staged in its LDS area, 8 items per lane, a wave64
+// segmented scan; sums of rows completed in the tile go to partial[pair], the
+// tile's trailing partial and a deferred first row go to carry (k_hot_fix).
+template <typename T, int ADD, int MUL>
+__global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
+    const int32_t* __restrict__ vrp, const int32_t* __restrict__ hcol, const T* __restrict__ vval,
+    const T* __restrict__ x, const T* __restrict__ xh, int H, int nseg, T* __restrict__ partial,
+    const int2* __restrict__ tiles, const int64_t* __restrict__ seg_tile, T* __restrict__ carry, int64_t npairs) {
+    constexpr int WI = HOT_WT / 64;
+    extern __shared__ __align__(16) unsigned char hot_lds[];
+    T* s_x = (T*)hot_lds;
+    int32_t* s_end = (int32_t*)(hot_lds + (size_t)H * sizeof(T)) + (threadIdx.x >> 6) * (HOT_WT + 1);
+    T* s_prod = (T*)((int32_t*)(hot_lds + (size_t)H * sizeof(T)) + HOT_WAVES * (HOT_WT + 1)) + (threadIdx.x >> 6) * HOT_WT;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+    const T ident = add_identity<ADD, T>();
+    for (int seg = blockIdx.x & 7; seg < nseg; seg += 8) {
+        const T* src = xh + (int64_t)seg * H;
+        for (int i = threadIdx.x; i < H; i += HOT_WAVES * 64) s_x[i] = src[i];
+        __syncthreads();
+        const int64_t t_hi = seg_tile[seg + 1];
+        for (int64_t tile = seg_tile[seg] + (int64_t)j * HOT_WAVES + wid; tile < t_hi; tile += (int64_t)nbx * HOT_WAVES) {
+            const int2 c0 = tiles[tile], c1 = tiles[tile + 1];
+            const int r0 = c0.x, k0 = c0.y, nr = c1.x - r0, nk = c1.y - k0;
+            for (int i = lane; i <= nr; i += 64) {
+                const int r = r0 + 1 + i;
+                s_end[i] = (r <= npairs) ? vrp[r] - k0 : nk;
+            }
+            int ci[WI];
+            T av[WI], xv[WI];
+#pragma unroll
+            for (int it = 0; it < WI; ++it) {
+                const int q = it * 64 + lane;
+                ci[it] = 0;
+                av[it] = T(0);
+                if (q < nk) {
+                    ci[it] = __builtin_nontemporal_load(hcol + k0 + q);
+                    av[it] = __builtin_nontemporal_load(vval + k0 + q);
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < WI; ++it) {
+                const int q = it * 64 + lane;
+                xv[it] = T(0);
+                if (q < nk) xv[it] = ci[it] < 0 ? s_x[-1 - ci[it]] : x[ci[it]];
+            }
+#pragma unroll
+            for (int it = 0; it < WI; ++it) {
+                const int q = it * 64 + lane;
+                if (q < nk) s_prod[q] = mul_op<MUL, T>(av[it], xv[it]);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            const int total = nr + nk;
+            const int d = min(lane * WI, total);
+            int lo = max(d - nk, 0), hi = min(d, nr);
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_end[mid] <= d - mid - 1) lo = mid + 1;
+                else hi = mid;
+            }
+            int ri = lo, kj = d - lo;
+            const int dend = min(d + WI, total);
+            T acc = ident, firstsum = ident;
+            int first = -1;
+            for (int it = d; it < dend; ++it) {
+                if (kj < s_end[ri]) {
+                    acc = add_op<ADD, T>(acc, s_prod[kj]);
+                    ++kj;
+                } else {
+                    if (first < 0) {
+                        first = ri;
+                        firstsum = acc;
+                    } else {
+                        partial[r0 + ri] = acc;
+                    }
+                    acc = ident;
+                    ++ri;
+                }
+            }
+            // segmented scan of (trailing partial, has-row-end) over the wave
+            T v = acc;
+            int f = first >= 0;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const T ov = hot_shfl_up(v, off);
+                const int of = __shfl_up(f, off, 64);
+                if (lane >= off) {
+                    if (!f) v = add_op<ADD, T>(ov, v);
+                    f |= of;
+                }
+            }
+            T ev = hot_shfl_up(v, 1);
+            if (lane == 0) ev = ident;
+            const T tail = hot_shfl(v, 63);
+            const bool defer0 = (nr > 0) && (vrp[r0] < k0);
+            if (first >= 0) {
+                const T tot = add_op<ADD, T>(ev, firstsum);
+                if (first == 0 && defer0) carry[2 * tile + 1] = tot;
+                else partial[r0 + first] = tot;
+            }
+            if (lane == 0) carry[2 * tile] = tail;
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+    }
+}
+
+// Rows that began in an earlier tile: their sum is the deferred part plus the
+// trailing partials of the tiles back to the one holding the row start, in
+// tile order.
+template <typename T, int ADD>
+__global__ void k_hot_fix(const int32_t* __restrict__ vrp, const int2* __restrict__ tiles, int64_t ntiles,
+                          const T* __restrict__ carry, T* __restrict__ partial) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int2 c0 = tiles[t], c1 = tiles[t + 1];
+    const int r0 = c0.x, k0 = c0.y;
+    if (c1.x == r0) return;
+    const int start = vrp[r0];
+    if (start >= k0) return;
+    T total = carry[2 * t + 1];
+    for (int64_t u = t - 1; u >= 0; --u) {
+        total = add_op<ADD, T>(carry[2 * u], total);
+        if (tiles[u].y <= start) break;
+    }
+    partial[r0] = total;
+}
+
+template <typename T, int ADD, int MUL> static int hot_prepare() {
+    // the dynamic LDS of the persistent kernel exceeds the 64 KiB default
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        err = hipFuncSetAttribute((const void*)k_seg_hot<T, ADD, MUL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)device_lds_bytes());
+    });
+    if (err != hipSuccess) return fail(GBK_PANIC, "hipFuncSetAttribute: %s", hipGetErrorString(err));
+    return GBK_SUCCESS;
+}
+
+template <typename T, int ADD, int MUL>
+static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
+    const SegHot& h = s->hot;
+    hipStream_t st = stream();
+    const int64_t nh = (int64_t)s->nseg * h.H;
+    if (nh > 0) {
+        hipLaunchKernelGGL((k_hot_x<T, ADD>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, h.hot_ids, x,
+                           (T*)h.xh, nh);
+        GBK_LAUNCH_CHECK();
+    }
+    if (h.ntiles > 0) {
+        GBK_TRY((hot_prepare<T, ADD, MUL>()));
+        const size_t lds = (size_t)h.H * sizeof(T) + hot_staging_bytes(sizeof(T));
+        const int grid = 8 * std::max(1, device_cu_count() / 8);
+        hipLaunchKernelGGL((k_seg_hot<T, ADD, MUL>), dim3(grid), dim3(HOT_WAVES * 64), lds, st, s->v.row_ptr, h.hcol,
+                           (const T*)s->v.val, x, (const T*)h.xh, h.H, s->nseg, (T*)s->partial, h.tiles, h.seg_tile,
+                           (T*)h.carry, s->npairs);
+        GBK_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_hot_fix<T, ADD>), dim3((unsigned)((h.ntiles + 255) / 256)), dim3(256), 0, st,
+                           s->v.row_ptr, h.tiles, h.ntiles, (const T*)h.carry, (T*)s->partial);
+        GBK_LAUNCH_CHECK();
+    }
+    if (s->n > 0) {
+        hipLaunchKernelGGL((k_seg_merge<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st, s->rpp,
+                           s->row_pairs, (const T*)s->partial, y, s->n, accum);
+        GBK_LAUNCH_CHECK();
+    }
+    return GBK_SUCCESS;
+}
+
 static int seg_spmv(const SegCSR* s, int type, const void* x, void* y, int add, int mul, int accum) {
+    if (s->hot.tiles && !order_sensitive_monoid(add)) {
+        return dispatch_type(type, [&](auto tv) -> int {
+            typedef decltype(tv) T;
+            return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+                constexpr int ADD = decltype(a)::value, MUL = decltype(b)::value;
+                return seg_hot_spmv<T, ADD, MUL>(s, (const T*)x, (T*)y, accum);
+            });
+        });
+    }
     GBK_TRY(spmv_merge_mapped(s->v, type, x, s->partial, add, mul, GBK_ACC_NONE, s->tile_map));
     return dispatch_type(type, [&](auto tv) -> int {
         typedef decltype(tv) T;
@@ -445,12 +711,17 @@ int format_spmv(gbk_matrix A, int transposed, const void* x, void* y, int add, i
         *handled = true;
         return sell_spmv(A->sell[t], m, A->type, x, y, add, mul, accum);
     }
+    // Order-sensitive monoids (less, first, second) are not associative: a
+    // fold split by column segments would depend on the segmentation, so they
+    // take the CSR left fold (*handled stays false).
     if (A->format == GBK_FMT_CSR_SEG && A->seg[t]) {
+        if (order_sensitive_monoid(add)) return GBK_SUCCESS;
         *handled = true;
         return seg_spmv(A->seg[t], A->type, x, y, add, mul, accum);
     }
     if (A->format == GBK_FMT_LAV && A->lav[t]) {
         if (transposed) return fail(GBK_INVALID_VALUE, "unsupported matrix storage format in VSpM");
+        if (order_sensitive_monoid(add)) return GBK_SUCCESS;
         *handled = true;
         return lav_spmv(A->lav[t], A->type, x, y, add, mul, accum);
     }
@@ -466,6 +737,9 @@ int64_t format_bytes(gbk_matrix A) {
                            s->lng.nnz * (4 + vs) + s->lng.nrows * 4);
         const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
         if (sg) b += (int64_t)(sg->npairs * (4 + 4 + vs + 4) + sg->v.nnz * (4 + vs) + sg->n * 4 + sg->v.ntiles * 28);
+        if (sg && sg->hot.tiles)
+            b += (int64_t)(sg->v.nnz * 4 + (int64_t)sg->nseg * sg->hot.H * (4 + vs) + sg->hot.ntiles * (8 + 2 * vs) +
+                           (sg->nseg + 1) * 8);
         if (A->lav[t]) b += A->lav[t]->seg.n * (4 + vs);
     }
     return b;

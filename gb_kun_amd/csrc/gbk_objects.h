@@ -124,6 +124,7 @@ void set_thread_stream(hipStream_t s);
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
 int device_cu_count();
+int64_t device_lds_bytes();  // LDS a workgroup may allocate
 // scratch buffers owned by the library (grown on demand, reused across calls)
 int scratch(int slot, size_t bytes, void** p);
 

@@ -42,6 +42,7 @@ static int g_device = 0;
 static hipStream_t g_own_stream = nullptr;
 static hipStream_t g_stream = nullptr;
 static int g_cus = 0;
+static int64_t g_lds = 0;
 
 struct ScratchSlot { void* p = nullptr; size_t bytes = 0; };
 static ScratchSlot g_scratch[16];
@@ -60,6 +61,7 @@ int init_device(int device) {
     hipDeviceProp_t prop;
     GBK_HIP(hipGetDeviceProperties(&prop, device));
     g_cus = prop.multiProcessorCount;
+    g_lds = (int64_t)prop.sharedMemPerBlock;
     g_device = device;
     g_init = true;
     return GBK_SUCCESS;
@@ -81,6 +83,7 @@ static thread_local hipStream_t t_stream = nullptr;
 hipStream_t stream() { return t_stream ? t_stream : g_stream; }
 void set_thread_stream(hipStream_t s) { t_stream = s; }
 int device_cu_count() { return g_cus; }
+int64_t device_lds_bytes() { return g_lds; }
 
 int dev_alloc(void** p, size_t bytes) {
     *p = nullptr;

@@ -89,3 +89,54 @@ def test_problematic_chunks_not_identity(gbk, oracle):
     w = gbk.Vector(n, gbk.INT32)
     gbk.mxv(w, None, gbk.ACC_NONE, "PlusMultiplies", A, gbk.Vector.dense(x))
     assert np.array_equal(w.to_numpy(), oracle.spmv(rp, col, vals, x, np.zeros(n, np.int32), 0, 0))
+
+
+def _seg_case(gbk, oracle, rp, col, vals, x, sr, exact=True):
+    A = gbk.Matrix.from_csr(rp, col, vals, fmt=3)
+    n = A.n
+    add, mul = gbk.SEMIRINGS[sr]
+    w = gbk.Vector(n, gbk.TYPE_OF[vals.dtype])
+    gbk.mxv(w, None, gbk.ACC_NONE, sr, A, gbk.Vector.dense(x))
+    rp2 = np.concatenate([rp, np.full(n + 1 - len(rp), rp[-1])])
+    want = oracle.spmv(rp2, col, vals, x, np.zeros(n, vals.dtype), add, mul)
+    got = w.to_numpy()
+    A.free()
+    return got, want
+
+
+def test_seg_hot_columns_every_type(gbk, oracle):
+    """The segmented format's hot-column kernel (x of a segment's highest
+    in-degree columns in LDS) for every value type; order-sensitive monoids
+    take the sequential path.  R-MAT 18 has more columns per segment than LDS
+    slots, so hot and L2 columns mix in every tile."""
+    graphs = {"rmat18": gbk.gen_rmat(18, 8, seed=61), "rmat12": gbk.gen_rmat(12, 16, seed=62)}
+    graphs.update(special_graphs())
+    for gname, (rp, col) in graphs.items():
+        nnz = len(col)
+        n = len(rp) - 1
+        u = gbk.uniform_f32(63, nnz)
+        ux = gbk.uniform_f32(64, n)
+        for dt, sems in ((np.int32, ("PlusMultiplies", "MinimumPlus", "CustomLessPlus", "MinimumSelectSecond",
+                                     "PlusOne", "MinimumNotEqualTo")),
+                         (np.int64, ("PlusMultiplies", "MinimumPlus")),
+                         (np.uint8, ("LogicalOrAnd", "StructuralBool"))):
+            if dt == np.uint8:
+                vals = (u < 0.7).astype(np.uint8)
+                x = (ux < 0.3).astype(np.uint8)
+            else:
+                vals = (1 + (u * 255).astype(np.int64)).astype(dt)
+                x = (ux * 1000).astype(dt)
+            for sr in sems:
+                got, want = _seg_case(gbk, oracle, rp, col, vals, x, sr)
+                assert np.array_equal(got, want), f"{gname} {dt.__name__} {sr}"
+        for dt in (np.float32, np.float64):
+            vals = u.astype(dt)
+            x = ux.astype(dt)
+            got, _ = _seg_case(gbk, oracle, rp, col, vals, x, "PlusMultiplies")
+            rp2 = np.concatenate([rp, np.full(n + 1 - len(rp), rp[-1])])
+            y64, absum = oracle.spmv_f64(rp2, col, vals.astype(np.float32), x.astype(np.float32))
+            # fp64 inputs are fp32-representable, so fp64 products are exact and
+            # the bound is fp64 summation rounding
+            tol = 1e-5 if dt == np.float32 else 1e-10
+            err = np.abs(got.astype(np.float64) - y64)
+            assert np.all(err <= tol * absum + 1e-30), f"{gname} {dt.__name__}"
