@@ -327,11 +327,13 @@ static int64_t seg_cols_default(int64_t n) {
 }
 
 // Persistent hot-column kernel shape: 16 waves per block, each wave merges
-// tiles of HOT_WT items (8 per lane) on its own LDS staging area.
+// tiles of HOT_WT items (8 per lane) on its own LDS staging area (a row-end
+// bitmap of HOT_WT bits and HOT_WT products).
 constexpr int HOT_WAVES = 16;
 constexpr int HOT_WT = 512;
+constexpr int HOT_MW = HOT_WT / 32;
 
-static size_t hot_staging_bytes(size_t vs) { return (size_t)HOT_WAVES * ((HOT_WT + 1) * 4 + HOT_WT * vs); }
+static size_t hot_staging_bytes(size_t vs) { return (size_t)HOT_WAVES * (HOT_MW * 4 + HOT_WT * vs); }
 
 // LDS slots per segment for values of vs bytes: what one workgroup's LDS
 // holds beside the staging areas, a multiple of 64.
@@ -421,20 +423,25 @@ __global__ void k_hot_x(const int32_t* __restrict__ ids, const T* __restrict__ x
 // runs on XCD b % 8 and serves segments b % 8, b % 8 + 8, ...; within a
 // segment, wave w of block j takes tiles seg_tile[s] + j * HOT_WAVES + w in
 // steps of (grid / 8) * HOT_WAVES.  Each wave merges its 512-item tile alone:
-// row ends and products staged in its LDS area, 8 items per lane, a wave64
-// segmented scan; sums of rows completed in the tile go to partial[pair], the
-// tile's trailing partial and a deferred first row go to carry (k_hot_fix).
+// the tile's row ends are marked in a 512-bit mask at their merge positions
+// (row i ends at item i + its end offset), so lane L reads its 8 item kinds
+// from the mask, starts at row (marks before item 8L) and consumes the next
+// 8 - (its marks) products, all read before its walk; a wave64 segmented
+// scan joins rows split across lanes.  Sums of rows completed in the tile go
+// to partial[pair], the tile's trailing partial and a deferred first row go
+// to carry (k_hot_fix).
 template <typename T, int ADD, int MUL>
 __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
     const int32_t* __restrict__ vrp, const int32_t* __restrict__ hcol, const T* __restrict__ vval,
     const T* __restrict__ x, const T* __restrict__ xh, int H, int nseg, T* __restrict__ partial,
-    const int2* __restrict__ tiles, const int64_t* __restrict__ seg_tile, T* __restrict__ carry, int64_t npairs) {
-    constexpr int WI = HOT_WT / 64;
+    const int2* __restrict__ tiles, const int64_t* __restrict__ seg_tile, T* __restrict__ carry) {
+    constexpr int WI = HOT_WT / 64, LPW = 32 / WI;
+    static_assert(WI * 64 == HOT_WT && 32 % WI == 0, "lane bits must tile a mask word");
     extern __shared__ __align__(16) unsigned char hot_lds[];
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_x = (T*)hot_lds;
-    int32_t* s_end = (int32_t*)(hot_lds + (size_t)H * sizeof(T)) + (threadIdx.x >> 6) * (HOT_WT + 1);
-    T* s_prod = (T*)((int32_t*)(hot_lds + (size_t)H * sizeof(T)) + HOT_WAVES * (HOT_WT + 1)) + (threadIdx.x >> 6) * HOT_WT;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t* s_mask = (uint32_t*)(hot_lds + (size_t)H * sizeof(T)) + wid * HOT_MW;
+    T* s_prod = (T*)((uint32_t*)(hot_lds + (size_t)H * sizeof(T)) + HOT_WAVES * HOT_MW) + wid * HOT_WT;
     const int j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
     const T ident = add_identity<ADD, T>();
     for (int seg = blockIdx.x & 7; seg < nseg; seg += 8) {
@@ -445,9 +452,13 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
         for (int64_t tile = seg_tile[seg] + (int64_t)j * HOT_WAVES + wid; tile < t_hi; tile += (int64_t)nbx * HOT_WAVES) {
             const int2 c0 = tiles[tile], c1 = tiles[tile + 1];
             const int r0 = c0.x, k0 = c0.y, nr = c1.x - r0, nk = c1.y - k0;
-            for (int i = lane; i <= nr; i += 64) {
-                const int r = r0 + 1 + i;
-                s_end[i] = (r <= npairs) ? vrp[r] - k0 : nk;
+            // merge position of each row end in the tile (row i < nr ends inside it)
+            int pos[WI];
+#pragma unroll
+            for (int m = 0; m < WI; ++m) {
+                const int i = m * 64 + lane;
+                pos[m] = -1;
+                if (i < nr) pos[m] = i + (vrp[r0 + 1 + i] - k0);
             }
             int ci[WI];
             T av[WI], xv[WI];
@@ -461,44 +472,63 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
                     av[it] = __builtin_nontemporal_load(vval + k0 + q);
                 }
             }
+            if (lane < HOT_MW) s_mask[lane] = 0u;
 #pragma unroll
             for (int it = 0; it < WI; ++it) {
                 const int q = it * 64 + lane;
                 xv[it] = T(0);
                 if (q < nk) xv[it] = ci[it] < 0 ? s_x[-1 - ci[it]] : x[ci[it]];
             }
+            // LDS operations of one wave complete in issue order: the mask is
+            // cleared before it is marked, and written before it is read
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int m = 0; m < WI; ++m)
+                if (pos[m] >= 0) atomicOr(&s_mask[pos[m] >> 5], 1u << (pos[m] & 31));
 #pragma unroll
             for (int it = 0; it < WI; ++it) {
                 const int q = it * 64 + lane;
                 if (q < nk) s_prod[q] = mul_op<MUL, T>(av[it], xv[it]);
             }
-            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             const int total = nr + nk;
-            const int d = min(lane * WI, total);
-            int lo = max(d - nk, 0), hi = min(d, nr);
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_end[mid] <= d - mid - 1) lo = mid + 1;
-                else hi = mid;
+            const int w = lane / LPW, sh = WI * (lane % LPW);
+            int before = 0;
+#pragma unroll
+            for (int q = 0; q < HOT_MW; ++q) {
+                const uint32_t mw = s_mask[q];
+                before += (q < w) ? __builtin_popcount(mw) : 0;
             }
-            int ri = lo, kj = d - lo;
-            const int dend = min(d + WI, total);
+            const uint32_t mine = s_mask[w];
+            before += __builtin_popcount(mine & ((1u << sh) - 1u));
+            const int nitems = min(max(total - lane * WI, 0), WI);
+            const uint32_t bits = (mine >> sh) & ((1u << nitems) - 1u);
+            const int kj = lane * WI - before;
+            const int nnz_l = nitems - __builtin_popcount(bits);
+            T pv[WI];
+#pragma unroll
+            for (int m = 0; m < WI; ++m) pv[m] = (m < nnz_l) ? s_prod[kj + m] : ident;
             T acc = ident, firstsum = ident;
-            int first = -1;
-            for (int it = d; it < dend; ++it) {
-                if (kj < s_end[ri]) {
-                    acc = add_op<ADD, T>(acc, s_prod[kj]);
-                    ++kj;
-                } else {
-                    if (first < 0) {
-                        first = ri;
-                        firstsum = acc;
+            int first = -1, ri = before;
+#pragma unroll
+            for (int m = 0; m < WI; ++m) {
+                if (m < nitems) {
+                    if ((bits >> m) & 1u) {
+                        if (first < 0) {
+                            first = ri;
+                            firstsum = acc;
+                        } else {
+                            partial[r0 + ri] = acc;
+                        }
+                        acc = ident;
+                        ++ri;
                     } else {
-                        partial[r0 + ri] = acc;
+                        acc = add_op<ADD, T>(acc, pv[0]);
+#pragma unroll
+                        for (int k = 0; k < WI - 1; ++k) pv[k] = pv[k + 1];
                     }
-                    acc = ident;
-                    ++ri;
                 }
             }
             // segmented scan of (trailing partial, has-row-end) over the wave
@@ -523,7 +553,7 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
                 else partial[r0 + first] = tot;
             }
             if (lane == 0) carry[2 * tile] = tail;
-            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
@@ -578,8 +608,8 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
         const size_t lds = (size_t)h.H * sizeof(T) + hot_staging_bytes(sizeof(T));
         const int grid = 8 * std::max(1, device_cu_count() / 8);
         hipLaunchKernelGGL((k_seg_hot<T, ADD, MUL>), dim3(grid), dim3(HOT_WAVES * 64), lds, st, s->v.row_ptr, h.hcol,
-                           (const T*)s->v.val, x, (const T*)h.xh, h.H, s->nseg, (T*)s->partial, h.tiles, h.seg_tile,
-                           (T*)h.carry, s->npairs);
+                           (const T*)s->v.val, x, (const T*)h.xh, h.H, s->nseg, (T*)s->partial, h.tiles,
+                           h.seg_tile, (T*)h.carry);
         GBK_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_hot_fix<T, ADD>), dim3((unsigned)((h.ntiles + 255) / 256)), dim3(256), 0, st,
                            s->v.row_ptr, h.tiles, h.ntiles, (const T*)h.carry, (T*)s->partial);
