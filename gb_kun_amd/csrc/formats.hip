@@ -327,13 +327,12 @@ static int64_t seg_cols_default(int64_t n) {
 }
 
 // Persistent hot-column kernel shape: 16 waves per block, each wave merges
-// tiles of HOT_WT items (8 per lane) on its own LDS staging area (a row-end
-// bitmap of HOT_WT bits and HOT_WT products).
+// tiles of HOT_WT items (8 per lane) on its own LDS staging area (one
+// item-kind byte per item and HOT_WT products).
 constexpr int HOT_WAVES = 16;
 constexpr int HOT_WT = 512;
-constexpr int HOT_MW = HOT_WT / 32;
 
-static size_t hot_staging_bytes(size_t vs) { return (size_t)HOT_WAVES * (HOT_MW * 4 + HOT_WT * vs); }
+static size_t hot_staging_bytes(size_t vs) { return (size_t)HOT_WAVES * (HOT_WT + HOT_WT * vs); }
 
 // LDS slots per segment for values of vs bytes: what one workgroup's LDS
 // holds beside the staging areas, a multiple of 64.
@@ -423,27 +422,29 @@ __global__ void k_hot_x(const int32_t* __restrict__ ids, const T* __restrict__ x
 // runs on XCD b % 8 and serves segments b % 8, b % 8 + 8, ...; within a
 // segment, wave w of block j takes tiles seg_tile[s] + j * HOT_WAVES + w in
 // steps of (grid / 8) * HOT_WAVES.  Each wave merges its 512-item tile alone:
-// the tile's row ends are marked in a 512-bit mask at their merge positions
-// (row i ends at item i + its end offset), so lane L reads its 8 item kinds
-// from the mask, starts at row (marks before item 8L) and consumes the next
-// 8 - (its marks) products, all read before its walk; a wave64 segmented
-// scan joins rows split across lanes.  Sums of rows completed in the tile go
-// to partial[pair], the tile's trailing partial and a deferred first row go
-// to carry (k_hot_fix).
+// row i of the tile ends at merge item i + (its end offset), marked by a byte
+// in the wave's item-kind array (distinct bytes, no atomics); lane L reads
+// its 8 item kinds, its first row is a wave prefix sum of the per-lane
+// row-end counts (from four ballots), its item m is its
+// (m - row ends below m)-th nonzero, whose product is read from LDS before
+// the fold; a wave64 segmented scan joins rows split across lanes.  Sums of
+// rows completed in the tile go to partial[pair], the tile's trailing
+// partial and a deferred first row go to carry (k_hot_fix).
 template <typename T, int ADD, int MUL>
 __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
     const int32_t* __restrict__ vrp, const int32_t* __restrict__ hcol, const T* __restrict__ vval,
     const T* __restrict__ x, const T* __restrict__ xh, int H, int nseg, T* __restrict__ partial,
     const int2* __restrict__ tiles, const int64_t* __restrict__ seg_tile, T* __restrict__ carry) {
-    constexpr int WI = HOT_WT / 64, LPW = 32 / WI;
-    static_assert(WI * 64 == HOT_WT && 32 % WI == 0, "lane bits must tile a mask word");
+    constexpr int WI = HOT_WT / 64;
+    static_assert(WI == 8, "eight item-kind bytes per lane");
     extern __shared__ __align__(16) unsigned char hot_lds[];
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_x = (T*)hot_lds;
-    uint32_t* s_mask = (uint32_t*)(hot_lds + (size_t)H * sizeof(T)) + wid * HOT_MW;
-    T* s_prod = (T*)((uint32_t*)(hot_lds + (size_t)H * sizeof(T)) + HOT_WAVES * HOT_MW) + wid * HOT_WT;
+    uint8_t* s_kind = hot_lds + (size_t)H * sizeof(T) + wid * HOT_WT;
+    T* s_prod = (T*)(hot_lds + (size_t)H * sizeof(T) + HOT_WAVES * HOT_WT) + wid * HOT_WT;
     const int j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
     const T ident = add_identity<ADD, T>();
+    const uint64_t below = (1ull << lane) - 1ull;
     for (int seg = blockIdx.x & 7; seg < nseg; seg += 8) {
         const T* src = xh + (int64_t)seg * H;
         for (int i = threadIdx.x; i < H; i += HOT_WAVES * 64) s_x[i] = src[i];
@@ -452,16 +453,13 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
         for (int64_t tile = seg_tile[seg] + (int64_t)j * HOT_WAVES + wid; tile < t_hi; tile += (int64_t)nbx * HOT_WAVES) {
             const int2 c0 = tiles[tile], c1 = tiles[tile + 1];
             const int r0 = c0.x, k0 = c0.y, nr = c1.x - r0, nk = c1.y - k0;
-            // merge position of each row end in the tile (row i < nr ends inside it)
-            int pos[WI];
+            // row ends: unconditional loads of clamped (in-range) indices, so
+            // that all of them are in flight together
+            int rend[WI];
 #pragma unroll
-            for (int m = 0; m < WI; ++m) {
-                const int i = m * 64 + lane;
-                pos[m] = -1;
-                if (i < nr) pos[m] = i + (vrp[r0 + 1 + i] - k0);
-            }
+            for (int m = 0; m < WI; ++m) rend[m] = vrp[(uint32_t)min(r0 + 1 + m * 64 + lane, r0 + nr)];
             int ci[WI];
-            T av[WI], xv[WI];
+            T av[WI];
 #pragma unroll
             for (int it = 0; it < WI; ++it) {
                 const int q = it * 64 + lane;
@@ -472,20 +470,23 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
                     av[it] = __builtin_nontemporal_load(vval + k0 + q);
                 }
             }
-            if (lane < HOT_MW) s_mask[lane] = 0u;
+            *(uint2*)(s_kind + lane * WI) = make_uint2(0u, 0u);
+            T xv[WI];
 #pragma unroll
             for (int it = 0; it < WI; ++it) {
                 const int q = it * 64 + lane;
                 xv[it] = T(0);
-                if (q < nk) xv[it] = ci[it] < 0 ? s_x[-1 - ci[it]] : x[ci[it]];
+                if (q < nk) xv[it] = ci[it] < 0 ? s_x[-1 - ci[it]] : x[(uint32_t)ci[it]];
             }
-            // LDS operations of one wave complete in issue order: the mask is
-            // cleared before it is marked, and written before it is read
+            // LDS operations of one wave complete in issue order: the kinds
+            // are cleared before they are marked, and written before they are read
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int m = 0; m < WI; ++m)
-                if (pos[m] >= 0) atomicOr(&s_mask[pos[m] >> 5], 1u << (pos[m] & 31));
+            for (int m = 0; m < WI; ++m) {
+                const int i = m * 64 + lane;
+                if (i < nr) s_kind[i + rend[m] - k0] = 1;
+            }
 #pragma unroll
             for (int it = 0; it < WI; ++it) {
                 const int q = it * 64 + lane;
@@ -494,42 +495,36 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             const int total = nr + nk;
-            const int w = lane / LPW, sh = WI * (lane % LPW);
+            const uint2 kw = *(const uint2*)(s_kind + lane * WI);
+            const uint64_t kb = ((uint64_t)kw.y << 32) | kw.x;
+            const int nitems = min(max(total - lane * WI, 0), WI);
+            uint32_t bits = 0;
+#pragma unroll
+            for (int m = 0; m < WI; ++m) bits |= (uint32_t)((kb >> (8 * m)) & 1ull) << m;
+            bits &= (1u << nitems) - 1u;
+            const int cnt = __builtin_popcount(bits);
             int before = 0;
 #pragma unroll
-            for (int q = 0; q < HOT_MW; ++q) {
-                const uint32_t mw = s_mask[q];
-                before += (q < w) ? __builtin_popcount(mw) : 0;
-            }
-            const uint32_t mine = s_mask[w];
-            before += __builtin_popcount(mine & ((1u << sh) - 1u));
-            const int nitems = min(max(total - lane * WI, 0), WI);
-            const uint32_t bits = (mine >> sh) & ((1u << nitems) - 1u);
+            for (int q = 0; q < 4; ++q) before += __builtin_popcountll(__ballot((cnt >> q) & 1) & below) << q;
             const int kj = lane * WI - before;
-            const int nnz_l = nitems - __builtin_popcount(bits);
             T pv[WI];
 #pragma unroll
-            for (int m = 0; m < WI; ++m) pv[m] = (m < nnz_l) ? s_prod[kj + m] : ident;
+            for (int m = 0; m < WI; ++m) pv[m] = s_prod[min(kj + m - __builtin_popcount(bits & ((1u << m) - 1u)), HOT_WT - 1)];
             T acc = ident, firstsum = ident;
             int first = -1, ri = before;
 #pragma unroll
             for (int m = 0; m < WI; ++m) {
-                if (m < nitems) {
-                    if ((bits >> m) & 1u) {
-                        if (first < 0) {
-                            first = ri;
-                            firstsum = acc;
-                        } else {
-                            partial[r0 + ri] = acc;
-                        }
-                        acc = ident;
-                        ++ri;
+                const bool isend = (bits >> m) & 1u;
+                if (isend) {
+                    if (first < 0) {
+                        first = ri;
+                        firstsum = acc;
                     } else {
-                        acc = add_op<ADD, T>(acc, pv[0]);
-#pragma unroll
-                        for (int k = 0; k < WI - 1; ++k) pv[k] = pv[k + 1];
+                        partial[r0 + ri] = acc;
                     }
+                    ++ri;
                 }
+                acc = isend ? ident : (m < nitems ? add_op<ADD, T>(acc, pv[m]) : acc);
             }
             // segmented scan of (trailing partial, has-row-end) over the wave
             T v = acc;
