@@ -80,7 +80,26 @@ __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, co
                                               int32_t nrows, int K, const int2* __restrict__ tiles, int64_t ntiles,
                                               T* __restrict__ carry_head, T* __restrict__ carry_tail, int accum) {
     constexpr int GPB = 256 / LW;
-    const int64_t t = (int64_t)blockIdx.x * GPB + threadIdx.x / LW;
+    // Row ends of the block's rows, staged in LDS by the whole block (eight
+    // loads in flight per thread), so that the walk below reads them there
+    // rather than one dependent global load per row.
+    __shared__ int32_t s_rend[GPB * SPMM_TILE + 1];
+    const int64_t tb0 = (int64_t)blockIdx.x * GPB;
+    const int64_t tb1 = min(tb0 + GPB, ntiles);
+    const int R0 = tiles[tb0].x;
+    const int nrend = tiles[tb1].x - R0 + 1;
+    for (int base = 0; base < nrend; base += 256 * 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = rp[(uint32_t)min(R0 + 1 + base + u * 256 + (int)threadIdx.x, nrows)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = base + u * 256 + (int)threadIdx.x;
+            if (i < nrend) s_rend[i] = v[u];
+        }
+    }
+    __syncthreads();
+    const int64_t t = tb0 + threadIdx.x / LW;
     if (t >= ntiles) return;  // whole group leaves together
     const int lane = threadIdx.x % LW;
     const bool act = lane < K;
@@ -91,7 +110,7 @@ __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, co
     int kk = c0.y;
     const int k_end = c1.y;
     const bool head_open = (row < row_lim) && (rp[row] < kk);
-    int row_end = (row < nrows) ? rp[row + 1] : k_end;
+    int row_end = s_rend[row - R0];
     T acc = ident;
 
     auto flush = [&](int r, T v) {
@@ -126,7 +145,7 @@ __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, co
                     flush(row, acc);
                     acc = ident;
                     ++row;
-                    row_end = rp[row + 1];
+                    row_end = s_rend[row - R0];
                 }
                 acc = add_op<ADD, T>(acc, mul_op<MUL, T>(a, xv[i]));
             }
