@@ -61,7 +61,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed loop, copy this many MiB device-to-device on the library stream "
                          "(a kernel of known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal"],
+    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal", "io"],
                     help="run one secondary measurement alone (for profiling); prints that section as JSON")
     args = ap.parse_args()
 
@@ -79,6 +79,8 @@ def main():
             res = batched_bench(g, rp, col, len(rp) - 1, args.scale)
         elif args.section == "served":
             res = served_bench(g, full=args.serve_full)
+        elif args.section == "io":
+            res = io_bench(g, args.scale, args.edge_factor)
         else:
             res = traversal_bench(g, args.bfs_scale)
         print(json.dumps({"section": args.section, args.section: res}), flush=True)
@@ -428,6 +430,40 @@ def served_bench(g, full=False, seed=11):
     for G in graphs:
         for m in G["m"]:
             m.free()
+    return res
+
+
+def io_bench(g, scale, ef):
+    """Edge-list files: the R-MAT graph written as .mtx and .mtxbin, then read
+    back on the host (gbk_mtx_read) and read into a device matrix with CSR and
+    CSC (Matrix::init_from_mtx)."""
+    import tempfile
+    rp, col = g.gen_rmat(scale, ef, 0.57, 0.19, 0.19, seed=1, symmetric=False, permute=True)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    cols = col.astype(np.int64)
+    res = {"graph": f"R-MAT scale {scale}, edge factor {ef}", "entries": len(cols)}
+    with tempfile.TemporaryDirectory() as d:
+        for suf in ("mtx", "mtxbin"):
+            p = os.path.join(d, "g." + suf)
+            t0 = time.perf_counter()
+            g.mtx_write(p, n, n, rows, cols)
+            tw = time.perf_counter() - t0
+            size = os.path.getsize(p)
+            t0 = time.perf_counter()
+            g.mtx_read(p)
+            tr = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            A = g.Matrix.from_mtx(p, g.FP32)
+            tb = time.perf_counter() - t0
+            ok = A.nnz == len(cols)
+            A.free()
+            os.remove(p)
+            res[suf] = {"bytes": size, "write_s": round(tw, 3), "read_s": round(tr, 3),
+                        "read_to_device_matrix_s": round(tb, 3),
+                        "read_Mentries_per_s": round(len(cols) / tr / 1e6, 1), "nnz_ok": ok}
+            log(f"{suf}: {size / 1e6:.0f} MB, write {tw:.2f} s, host read {tr:.2f} s "
+                f"({len(cols) / tr / 1e6:.0f} M entries/s), read into a device CSR + CSC {tb:.2f} s")
     return res
 
 

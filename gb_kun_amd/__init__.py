@@ -86,6 +86,9 @@ _sig("gbk_matrix_free", C.c_int, _vp)
 _sig("gbk_matrix_set_format", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
 _sig("gbk_matrix_get_format", C.c_int, _vp, C.POINTER(C.c_int))
 _sig("gbk_matrix_build_coo", C.c_int, _vp, _pi64, _pi64, _vp, _i64)
+_sig("gbk_matrix_read_mtx", C.c_int, _vp, C.c_char_p)
+_sig("gbk_mtx_read", C.c_int, C.c_char_p, _pi64, _pi64, _pi64, C.POINTER(_pi64), C.POINTER(_pi64))
+_sig("gbk_mtx_write", C.c_int, C.c_char_p, _i64, _i64, _pi64, _pi64, _i64)
 _sig("gbk_matrix_build_csr", C.c_int, _vp, _pi64, _pi64, _vp, _i64, _i64)
 _sig("gbk_matrix_build_csr32", C.c_int, _vp, _pi64, _pi32, _vp, _i64, _i64)
 _sig("gbk_matrix_nrows", C.c_int, _vp, _pi64)
@@ -200,6 +203,28 @@ def _take_csr(n, nnz, rp, col):
     return rp_np, col_np
 
 
+def mtx_read(path):
+    """(nrows, ncols, rows int64, cols int64) of a .mtx / .mtxbin file, 0-based, file order."""
+    nr, nc, nnz = _i64(0), _i64(0), _i64(0)
+    r, c = _pi64(), _pi64()
+    check(lib.gbk_mtx_read(os.fsencode(path), C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(r), C.byref(c)))
+    try:
+        k = nnz.value
+        rows = np.ctypeslib.as_array(r, shape=(k,)).copy() if k else np.zeros(0, np.int64)
+        cols = np.ctypeslib.as_array(c, shape=(k,)).copy() if k else np.zeros(0, np.int64)
+    finally:
+        lib.gbk_host_free(C.cast(r, C.c_void_p))
+        lib.gbk_host_free(C.cast(c, C.c_void_p))
+    return nr.value, nc.value, rows, cols
+
+
+def mtx_write(path, nrows, ncols, rows, cols):
+    """Writes a .mtx (pattern) or .mtxbin file; rows / cols 0-based."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    check(lib.gbk_mtx_write(os.fsencode(path), nrows, ncols, _ptr(rows, C.c_int64), _ptr(cols, C.c_int64), len(rows)))
+
+
 def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, symmetric=False, permute=True):
     """Seeded R-MAT graph as a CSR (row_ptr int64[n+1], col int32[nnz]): duplicates and self-loops removed."""
     n, nnz = _i64(0), _i64(0)
@@ -256,6 +281,15 @@ class Matrix:
         if fmt is not None:
             m.set_format(fmt, sell_c, sigma)
         m.build_csr(row_ptr, col, vals)
+        return m
+
+    @classmethod
+    def from_mtx(cls, path, type_=None, fmt=None):
+        """Matrix::init_from_mtx: a .mtx or .mtxbin file, every entry 1."""
+        m = cls(FP32 if type_ is None else type_)
+        if fmt is not None:
+            m.set_format(fmt)
+        check(lib.gbk_matrix_read_mtx(m.h, os.fsencode(path)))
         return m
 
     def set_format(self, fmt, sell_c=0, sigma=0):

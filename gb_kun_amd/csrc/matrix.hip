@@ -7,6 +7,7 @@
 // the other one (spmspv.h:126-146).
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 #include <omp.h>
 
@@ -169,6 +170,26 @@ extern "C" int gbk_matrix_build_csr(gbk_matrix A, const int64_t* row_ptr, const 
     return gbk_matrix_build_csr32(A, row_ptr, col.data(), vals, nrows, nnz);
 }
 
+// COO -> CSR on the host for an n x n matrix: a stable counting sort by row
+// keeps the list order inside each row.
+static int build_from_coo(gbk_matrix A, int64_t n, const int64_t* rows, const int64_t* cols, const void* vals,
+                          size_t val_stride, int64_t nvals) {
+    if (n >= (int64_t)INT32_MAX) return fail(GBK_INVALID_VALUE, "matrix dimension does not fit int32");
+    std::vector<int32_t> rp(n + 1, 0);
+    for (int64_t e = 0; e < nvals; ++e) rp[rows[e] + 1]++;
+    for (int64_t r = 0; r < n; ++r) rp[r + 1] += rp[r];
+    std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+    std::vector<int32_t> col(nvals);
+    const size_t vs = type_size(A->type);
+    std::vector<char> v(vs * nvals);
+    for (int64_t e = 0; e < nvals; ++e) {
+        int64_t o = pos[rows[e]]++;
+        col[o] = (int32_t)cols[e];
+        std::memcpy(&v[o * vs], (const char*)vals + e * val_stride, vs);
+    }
+    return build_common(A, n, rp.data(), col.data(), v.data());
+}
+
 extern "C" int gbk_matrix_build_coo(gbk_matrix A, const int64_t* rows, const int64_t* cols, const void* vals,
                                     int64_t nvals) {
     if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
@@ -183,22 +204,33 @@ extern "C" int gbk_matrix_build_coo(gbk_matrix A, const int64_t* rows, const int
         maxid = std::max(maxid, std::max(rows[e], cols[e]));
     }
     if (bad) return fail(GBK_INVALID_INDEX, "negative vertex id");
-    const int64_t n = maxid + 1;
-    if (n >= (int64_t)INT32_MAX) return fail(GBK_INVALID_VALUE, "matrix dimension does not fit int32");
-    // stable counting sort by row keeps the list order inside each row
-    std::vector<int32_t> rp(n + 1, 0);
-    for (int64_t e = 0; e < nvals; ++e) rp[rows[e] + 1]++;
-    for (int64_t r = 0; r < n; ++r) rp[r + 1] += rp[r];
-    std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
-    std::vector<int32_t> col(nvals);
-    const size_t vs = type_size(A->type);
-    std::vector<char> v(vs * nvals);
-    for (int64_t e = 0; e < nvals; ++e) {
-        int64_t o = pos[rows[e]]++;
-        col[o] = (int32_t)cols[e];
-        std::memcpy(&v[o * vs], (const char*)vals + e * vs, vs);
+    return build_from_coo(A, maxid + 1, rows, cols, vals, type_size(A->type), nvals);
+}
+
+namespace gbk {
+int mtx_read(const char* path, int64_t* nrows, int64_t* ncols, int64_t* nnz, int64_t** rows, int64_t** cols);
+}
+
+// Matrix::init_from_mtx (read_file.hpp:556-615): every entry gets the value
+// 1 (EDGE_VAL); the matrix is square of size max(rows, cols).
+extern "C" int gbk_matrix_read_mtx(gbk_matrix A, const char* path) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    int64_t nr = 0, nc = 0, nnz = 0;
+    int64_t *rp = nullptr, *cp = nullptr;
+    GBK_TRY(mtx_read(path, &nr, &nc, &nnz, &rp, &cp));
+    std::unique_ptr<int64_t, decltype(&std::free)> rows(rp, &std::free), cols(cp, &std::free);
+    GBK_TRY(check_nnz(nnz));
+    const int64_t n = std::max(nr, nc);
+    if (n == 0) return fail(GBK_NO_VALUE, "empty matrix file");
+    alignas(8) unsigned char one[8] = {0};
+    switch (A->type) {
+        case GBK_FP32: { float v = 1.0f; std::memcpy(one, &v, 4); break; }
+        case GBK_FP64: { double v = 1.0; std::memcpy(one, &v, 8); break; }
+        case GBK_INT32: { int32_t v = 1; std::memcpy(one, &v, 4); break; }
+        case GBK_INT64: { int64_t v = 1; std::memcpy(one, &v, 8); break; }
+        default: one[0] = 1; break;
     }
-    return build_common(A, n, rp.data(), col.data(), v.data());
+    return build_from_coo(A, n, rows.get(), cols.get(), one, 0, nnz);
 }
 
 extern "C" int gbk_matrix_nrows(gbk_matrix A, int64_t* n) {

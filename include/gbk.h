@@ -198,6 +198,13 @@ int gbk_matrix_build_csr(gbk_matrix A, const int64_t* row_ptr, const int64_t* co
 int gbk_matrix_build_csr32(gbk_matrix A, const int64_t* row_ptr, const int32_t* col_ids,
                            const void* vals, int64_t nrows, int64_t nnz);
 
+/* Matrix::init_from_mtx, matrix.hpp:107-110 / read_file.hpp:556-615: reads
+ * a Matrix Market (.mtx) or binary (.mtxbin) edge list (by suffix) and
+ * builds CSR and CSC; every entry gets the value 1 (the reference's
+ * EDGE_VAL); row order inside a row = file order; the matrix is square of
+ * size max(rows, cols) of the size line. */
+int gbk_matrix_read_mtx(gbk_matrix A, const char* path);
+
 int gbk_matrix_nrows(gbk_matrix A, int64_t* n);
 int gbk_matrix_ncols(gbk_matrix A, int64_t* n);
 int gbk_matrix_nvals(gbk_matrix A, int64_t* nnz);
@@ -466,6 +473,20 @@ int gbk_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint6
 int gbk_gen_uniform(int scale, int edge_factor, uint64_t seed, int symmetric,
                     int64_t* n, int64_t* nnz, int64_t** row_ptr, int32_t** col_ids);
 void gbk_host_free(void* p);
+
+/* Edge-list files on the host.  gbk_mtx_read replaces the readers of
+ * read_file.hpp (read_mtx_file_pipelined 52-170, binary_read_mtx_file
+ * 378-555): .mtx = '%' comment lines, a "rows cols entries" line, then one
+ * "row col [value]" line per entry (1-based, values ignored); .mtxbin =
+ * int64 rows, cols, entries, then int64 (row, col) pairs, 1-based.  Returns
+ * 0-based rows / cols in file order, allocated by the library (release with
+ * gbk_host_free).  Ids outside the declared size or a short file are errors.
+ * gbk_mtx_write writes either format (by suffix) in the given order, as
+ * EdgeListContainer::save_as_mtx (edges_container.hpp:33-88) and
+ * benchmark/scripts/mtx_api.py:61-90 lay them out. */
+int gbk_mtx_read(const char* path, int64_t* nrows, int64_t* ncols, int64_t* nnz, int64_t** rows, int64_t** cols);
+int gbk_mtx_write(const char* path, int64_t nrows, int64_t ncols, const int64_t* rows, const int64_t* cols,
+                  int64_t nnz);
 /* out[i] = uniform float in [0,1) from (seed, i). */
 void gbk_fill_uniform_f32(uint64_t seed, int64_t count, float* out);
 /* vals[e] = 1 / (number of entries in column col[e]) — PR transition values. */

@@ -309,6 +309,11 @@ public:
         if (row_ptrs == nullptr || col_ids == nullptr || values == nullptr) return GrB_NULL_POINTER;
         return detail::code(gbk_matrix_build_csr(h_, (const int64_t*)row_ptrs, (const int64_t*)col_ids, values, nrows, nvals));
     }
+    // matrix.hpp:107-110 / read_file.hpp:556-615: .mtx or .mtxbin by suffix;
+    // the reference throws on an unreadable file, so does this
+    void init_from_mtx(const std::string& mtx_name) {
+        if (gbk_matrix_read_mtx(h_, mtx_name.c_str()) != GBK_SUCCESS) throw gbk_last_error();
+    }
     LA_Info get_nrows(Index* n) const { int64_t v; LA_Info rc = detail::code(gbk_matrix_nrows(h_, &v)); *n = v; return rc; }
     LA_Info get_ncols(Index* n) const { return get_nrows(n); }
     LA_Info get_nvals(Index* n) const { int64_t v; LA_Info rc = detail::code(gbk_matrix_nvals(h_, &v)); *n = v; return rc; }

@@ -255,6 +255,17 @@ def ref_bfs(rp, col, source):
     return lv
 
 
+def ref_read_mtx(path):
+    """The compiled reference's Matrix::init_from_mtx: (row_ptr, col, seconds) of its CSR."""
+    nr, nnz, sec = C.c_int64(0), C.c_int64(0), C.c_double(0)
+    rc = ref().ref_mtx_load(os.fsencode(path), C.byref(nr), C.byref(nnz), C.byref(sec))
+    assert rc == 0
+    rp = np.empty(nr.value + 1, dtype=np.int64)
+    col = np.empty(max(nnz.value, 1), dtype=np.int64)
+    assert ref().ref_mtx_csr(_p(rp), _p(col)) == 0
+    return rp, col[:nnz.value], sec.value
+
+
 def ref_sssp(rp, col, w, source):
     rp = np.ascontiguousarray(rp, dtype=np.int64)
     col64 = np.ascontiguousarray(col, dtype=np.int64)

@@ -187,6 +187,36 @@ extern "C" int ref_pagerank(int64_t n, const int64_t* rp, const int64_t* col, in
     }
 }
 
+// Matrix::init_from_mtx (read_file.hpp:556-615) on a file; the CSR is kept
+// until the next call and copied out by ref_mtx_csr.
+static lablas::Matrix<float>* g_mtx = nullptr;
+
+extern "C" int ref_mtx_load(const char* path, int64_t* nrows, int64_t* nnz, double* seconds) {
+    try {
+        delete g_mtx;
+        g_mtx = new lablas::Matrix<float>;
+        const double t0 = omp_get_wtime();
+        g_mtx->init_from_mtx(path);
+        *seconds = omp_get_wtime() - t0;
+        const auto* csr = g_mtx->get_matrix()->get_csr();
+        *nrows = csr->get_num_rows();
+        *nnz = csr->get_nnz();
+        return 0;
+    } catch (const char* e) {
+        fprintf(stderr, "reference threw: %s\n", e);
+        return -1;
+    }
+}
+
+extern "C" int ref_mtx_csr(int64_t* rp, int64_t* col) {
+    if (!g_mtx) return -1;
+    const auto* csr = g_mtx->get_matrix()->get_csr();
+    const int64_t n = csr->get_num_rows();
+    for (int64_t i = 0; i <= n; ++i) rp[i] = csr->get_row_ptr()[i];
+    for (int64_t e = 0; e < rp[n]; ++e) col[e] = csr->get_col_ids()[e];
+    return 0;
+}
+
 extern "C" int ref_set_threads(int t) {
     omp_set_num_threads(t);
     return omp_get_max_threads();
