@@ -227,61 +227,6 @@ void build_segmented(int64_t n, const int32_t* rp, const int32_t* col, const voi
     }
 }
 
-void build_seg_hot(const SegHost& h, int64_t ncols, int H, int64_t wave_items, SegHotHost& out) {
-    out = SegHotHost();
-    const int nseg = h.nseg;
-    out.H = H;
-    // in-degree of every column over the stored entries
-    std::vector<int32_t> deg(ncols > 0 ? ncols : 1, 0);
-    for (int32_t c : h.col) deg[c]++;
-    std::vector<int32_t> slot(ncols > 0 ? ncols : 1, -1);
-    out.hot_ids.assign((size_t)nseg * (H > 0 ? H : 0), -1);
-    if (H > 0) {
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int s = 0; s < nseg; ++s) {
-            std::vector<int32_t> cs;
-            for (int64_t c = h.seg_bounds[s]; c < h.seg_bounds[s + 1]; ++c)
-                if (deg[c] >= 2) cs.push_back((int32_t)c);
-            // descending in-degree, ties by column id: the choice is deterministic
-            std::sort(cs.begin(), cs.end(), [&](int32_t a, int32_t b) { return deg[a] != deg[b] ? deg[a] > deg[b] : a < b; });
-            const int take = (int)std::min<size_t>(cs.size(), (size_t)H);
-            for (int i = 0; i < take; ++i) {
-                slot[cs[i]] = i;
-                out.hot_ids[(size_t)s * H + i] = cs[i];
-            }
-        }
-    }
-    out.hcol.resize(h.col.size());
-#pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < (int64_t)h.col.size(); ++e) {
-        const int32_t c = h.col[e];
-        out.hcol[e] = slot[c] >= 0 ? -1 - slot[c] : c;
-    }
-    // merge-path tiles per segment over its virtual rows [p0, p1)
-    out.seg_tile.assign(nseg + 1, 0);
-    const int32_t* vrp = h.vrp.data();
-    for (int s = 0; s < nseg; ++s) {
-        out.seg_tile[s] = (int64_t)out.tiles.size() / 2;
-        const int64_t p0 = h.seg_pair_start[s], p1 = h.seg_pair_start[s + 1];
-        const int64_t nr = p1 - p0, k_lo = vrp[p0], nk = (int64_t)vrp[p1] - k_lo;
-        const int64_t nt = (nr + nk + wave_items - 1) / wave_items;
-        for (int64_t i = 0; i < nt; ++i) {
-            const int64_t diag = std::min(i * wave_items, nr + nk);
-            int64_t lo = std::max<int64_t>(diag - nk, 0), hi = std::min<int64_t>(diag, nr);
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if ((int64_t)vrp[p0 + mid + 1] - k_lo <= diag - mid - 1) lo = mid + 1;
-                else hi = mid;
-            }
-            out.tiles.push_back((int32_t)(p0 + lo));
-            out.tiles.push_back((int32_t)(k_lo + diag - lo));
-        }
-    }
-    out.seg_tile[nseg] = (int64_t)out.tiles.size() / 2;
-    out.tiles.push_back((int32_t)h.npairs);
-    out.tiles.push_back((int32_t)h.col.size());
-}
-
 void lav_relabel(int64_t n, const int32_t* rp, const int32_t* col, double dense_frac, int64_t seg_cols,
                  std::vector<int32_t>& old_to_new, std::vector<int32_t>& new_to_old, std::vector<int64_t>& bounds) {
     std::vector<int64_t> deg(n, 0);

@@ -58,6 +58,7 @@ struct SegHot {
     int2* tiles = nullptr;        // ntiles + 1
     int64_t* seg_tile = nullptr;  // nseg + 1
     void* carry = nullptr;        // 2 * ntiles values
+    int32_t* seg_pair = nullptr;  // nseg + 1: first pair of each segment
 };
 
 struct SegCSR {
@@ -264,6 +265,7 @@ static void free_seg_fields(SegCSR& s) {
     dev_free(s.hot.tiles);
     dev_free(s.hot.seg_tile);
     dev_free(s.hot.carry);
+    dev_free(s.hot.seg_pair);
     s = SegCSR();
 }
 
@@ -344,41 +346,289 @@ static int hot_slots(size_t vs) {
     return h < 1024 ? 0 : (int)std::min<int64_t>(h, 1 << 20);
 }
 
-static int build_seg_hot_dev(const gbk_host::SegHost& h, int type, int64_t n, SegCSR& s) {
+// ------------------------------------------- segmented CSR, device build --
+// The (segment, row) pairs, the hot column slots and the wave tiles are built
+// on the device from the CSR already there: entries are stably sorted by
+// segment (CSR order is kept inside a segment), pairs start where the
+// (segment, row) key changes, pairs are stably sorted by row for the merge,
+// columns are sorted by (segment, descending in-degree, id) for the slots.
+// Replaces MatrixSegmentedCSR::build (seg_csr/build.hpp:24-169), which runs
+// single-threaded on the host.
+
+struct TmpBuf {
+    void* p = nullptr;
+    ~TmpBuf() { dev_free(p); }
+    int alloc(size_t bytes) { return dev_alloc(&p, bytes > 0 ? bytes : 1); }
+    template <typename V> V* as() const { return (V*)p; }
+};
+
+__global__ void k_seg_key(const int32_t* __restrict__ col, int64_t nnz, int64_t w, uint32_t* __restrict__ key) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) key[e] = (uint32_t)(col[e] / w);
+}
+
+// flag[i] = 1 where sorted entry i opens a new (segment, row) pair.
+__global__ void k_pair_flags(const uint32_t* __restrict__ skey, const int32_t* __restrict__ perm,
+                             const int32_t* __restrict__ rowid, int64_t nnz, int32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    flag[i] = (i == 0 || skey[i] != skey[i - 1] || rowid[perm[i]] != rowid[perm[i - 1]]) ? 1 : 0;
+}
+
+__global__ void k_pair_fill(const int32_t* __restrict__ flag, const int32_t* __restrict__ pidx,
+                            const uint32_t* __restrict__ skey, const int32_t* __restrict__ perm,
+                            const int32_t* __restrict__ rowid, int64_t nnz, int32_t* __restrict__ vrp,
+                            int32_t* __restrict__ pair_row, int32_t* __restrict__ pair_seg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz || !flag[i]) return;
+    const int32_t p = pidx[i];
+    vrp[p] = (int32_t)i;
+    pair_row[p] = rowid[perm[i]];
+    pair_seg[p] = (int32_t)skey[i];
+}
+
+__global__ void k_set_i32(int32_t* __restrict__ p, int64_t i, int32_t v) {
+    if (threadIdx.x == 0) p[i] = v;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Column sort key: segment in the high bits, then descending in-degree
+// (saturated), so a stable sort of column ids puts each segment's columns
+// together, most referenced first, ties by id.
+__global__ void k_hot_key(const int32_t* __restrict__ deg, int64_t n, int64_t w, int db, uint32_t* __restrict__ key) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t dmax = (1u << db) - 1u;
+    const uint32_t d = (uint32_t)min((int64_t)deg[c], (int64_t)dmax);
+    key[c] = ((uint32_t)(c / w) << db) | (dmax - d);
+}
+
+// Sorted position i of segment s holds its (i - s w)-th most referenced column.
+__global__ void k_hot_assign(const int32_t* __restrict__ order, const int32_t* __restrict__ deg, int64_t n, int64_t w,
+                             int H, int32_t* __restrict__ slot, int32_t* __restrict__ hot_ids) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = order[i];
+    const int64_t sg = c / w;
+    const int64_t rank = i - sg * w;
+    if (rank < H && deg[c] >= 2) {
+        slot[c] = (int32_t)rank;
+        hot_ids[sg * H + rank] = c;
+    }
+}
+
+__global__ void k_hot_encode(const int32_t* __restrict__ vcol, const int32_t* __restrict__ slot, int64_t nnz,
+                             int32_t* __restrict__ hcol) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const int32_t c = vcol[i];
+    const int32_t sl = slot[c];
+    hcol[i] = sl >= 0 ? -1 - sl : c;
+}
+
+// Merge-path start of every wave tile: tile t of segment s starts at
+// diagonal (t - seg_tile[s]) * HOT_WT of the segment's pairs [p0, p1).
+__global__ void k_seg_tiles(const int32_t* __restrict__ vrp, const int64_t* __restrict__ seg_tile,
+                            const int32_t* __restrict__ seg_pair, int nseg, int64_t ntiles, int64_t npairs,
+                            int64_t nnz, int2* __restrict__ tiles) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    if (t == ntiles) {
+        tiles[t] = make_int2((int)npairs, (int)nnz);
+        return;
+    }
+    int lo_s = 0, hi_s = nseg - 1;  // last segment with seg_tile[s] <= t
+    while (lo_s < hi_s) {
+        const int mid = (lo_s + hi_s + 1) >> 1;
+        if (seg_tile[mid] <= t) lo_s = mid;
+        else hi_s = mid - 1;
+    }
+    const int64_t p0 = seg_pair[lo_s], p1 = seg_pair[lo_s + 1];
+    const int64_t k_lo = vrp[p0], nr = p1 - p0, nk = (int64_t)vrp[p1] - k_lo;
+    const int64_t diag = min((t - seg_tile[lo_s]) * (int64_t)HOT_WT, nr + nk);
+    int64_t lo = max(diag - nk, (int64_t)0), hi = min(diag, nr);
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)vrp[p0 + mid + 1] - k_lo <= diag - mid - 1) lo = mid + 1;
+        else hi = mid;
+    }
+    tiles[t] = make_int2((int)(p0 + lo), (int)(k_lo + diag - lo));
+}
+
+static dim3 grid_e(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, SegCSR& s) {
+    hipStream_t st = stream();
     const size_t vs = type_size(type);
-    gbk_host::SegHotHost hh;
-    gbk_host::build_seg_hot(h, n, hot_slots(vs), HOT_WT, hh);
+    const int64_t n = m.nrows, nnz = m.nnz;
+    const int64_t z1 = nnz > 0 ? nnz : 1;
+    s.nseg = (int)nseg;
+    s.n = n;
+    // entries stably sorted by segment
+    TmpBuf key, skey, iota, perm, rowid, flag, pidx;
+    GBK_TRY(key.alloc(4 * z1));
+    GBK_TRY(skey.alloc(4 * z1));
+    GBK_TRY(iota.alloc(4 * z1));
+    GBK_TRY(perm.alloc(4 * z1));
+    GBK_TRY(rowid.alloc(4 * z1));
+    GBK_TRY(flag.alloc(4 * z1));
+    GBK_TRY(pidx.alloc(4 * (z1 + 1)));  // the scan also writes the total at [nnz]
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_seg_key, grid_e(nnz), dim3(256), 0, st, m.col, nnz, w, key.as<uint32_t>());
+        GBK_LAUNCH_CHECK();
+    }
+    GBK_TRY(dev_iota(iota.as<int32_t>(), nnz, st));
+    GBK_TRY(dev_sort_pairs(key.as<uint32_t>(), skey.as<uint32_t>(), iota.as<int32_t>(), perm.as<int32_t>(), nnz,
+                           bits_for(nseg - 1), st));
+    GBK_TRY(dev_row_ids(m.row_ptr, n, rowid.as<int32_t>(), st));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_pair_flags, grid_e(nnz), dim3(256), 0, st, skey.as<uint32_t>(), perm.as<int32_t>(),
+                           rowid.as<int32_t>(), nnz, flag.as<int32_t>());
+        GBK_LAUNCH_CHECK();
+    }
+    int32_t np32 = 0;
+    if (nnz > 0) GBK_TRY(exclusive_scan_i32(flag.as<int32_t>(), pidx.as<int32_t>(), nnz, &np32));
+    const int64_t npairs = np32;
+    s.npairs = npairs;
+    // the virtual CSR: one row per pair
+    TmpBuf pair_seg;
+    int32_t* pair_row = nullptr;
+    GBK_TRY(pair_seg.alloc(4 * (npairs > 0 ? npairs : 1)));
+    GBK_TRY(dev_alloc((void**)&pair_row, 4 * (npairs > 0 ? npairs : 1)));
+    TmpBuf pair_row_buf;
+    pair_row_buf.p = pair_row;  // released at the end of the build
+    s.v.nrows = s.v.ncols = npairs;
+    s.v.nnz = nnz;
+    GBK_TRY(dev_alloc((void**)&s.v.row_ptr, 4 * (npairs + 1)));
+    GBK_TRY(dev_alloc((void**)&s.v.col, 4 * z1));
+    GBK_TRY(dev_alloc(&s.v.val, vs * z1));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_pair_fill, grid_e(nnz), dim3(256), 0, st, flag.as<int32_t>(), pidx.as<int32_t>(),
+                           skey.as<uint32_t>(), perm.as<int32_t>(), rowid.as<int32_t>(), nnz, s.v.row_ptr, pair_row,
+                           pair_seg.as<int32_t>());
+        GBK_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(64), 0, st, s.v.row_ptr, npairs, (int32_t)nnz);
+    GBK_LAUNCH_CHECK();
+    GBK_TRY(dev_gather_i32(m.col, perm.as<int32_t>(), nnz, s.v.col, st));
+    GBK_TRY(dev_gather_bytes(m.val, perm.as<int32_t>(), nnz, vs, s.v.val, st));
+    // first pair of every segment
+    TmpBuf segcnt, segpair_d;
+    GBK_TRY(segcnt.alloc(4 * (nseg + 1)));
+    GBK_HIP(hipMemsetAsync(segcnt.p, 0, 4 * (nseg + 1), st));
+    GBK_TRY(dev_count(pair_seg.as<int32_t>(), npairs, segcnt.as<int32_t>(), st));
+    std::vector<int32_t> sc(nseg + 1);
+    GBK_HIP(hipMemcpyAsync(sc.data(), segcnt.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    std::vector<int32_t> seg_pair(nseg + 1, 0);
+    for (int64_t g = 0; g < nseg; ++g) seg_pair[g + 1] = seg_pair[g] + sc[g];
+    // rows -> their pairs in segment order: pairs stably sorted by row
+    TmpBuf piota, pkey_out;
+    GBK_TRY(piota.alloc(4 * (npairs > 0 ? npairs : 1)));
+    GBK_TRY(pkey_out.alloc(4 * (npairs > 0 ? npairs : 1)));
+    GBK_TRY(dev_alloc((void**)&s.row_pairs, 4 * (npairs > 0 ? npairs : 1)));
+    GBK_TRY(dev_alloc((void**)&s.rpp, 4 * (n + 1)));
+    GBK_TRY(dev_iota(piota.as<int32_t>(), npairs, st));
+    GBK_TRY(dev_sort_pairs((const uint32_t*)pair_row, pkey_out.as<uint32_t>(), piota.as<int32_t>(), s.row_pairs,
+                           npairs, bits_for(n > 0 ? n - 1 : 0), st));
+    {
+        TmpBuf rcnt;
+        GBK_TRY(rcnt.alloc(4 * (n + 1)));
+        GBK_HIP(hipMemsetAsync(rcnt.p, 0, 4 * (n + 1), st));
+        GBK_TRY(dev_count(pair_row, npairs, rcnt.as<int32_t>(), st));
+        int32_t tot = 0;
+        GBK_TRY(exclusive_scan_i32(rcnt.as<int32_t>(), s.rpp, n, &tot));  // rpp[0..n]
+    }
+    GBK_TRY(dev_alloc(&s.partial, vs * (npairs > 0 ? npairs : 1)));
+    // hot slots: columns by (segment, descending in-degree, id)
     SegHot& o = s.hot;
-    o.H = hh.H;
-    o.ntiles = (int64_t)hh.tiles.size() / 2 - 1;
-    GBK_TRY(upload(&o.hcol, hh.hcol));
-    GBK_TRY(upload(&o.hot_ids, hh.hot_ids));
-    GBK_TRY(dev_alloc(&o.xh, vs * (hh.hot_ids.empty() ? 1 : hh.hot_ids.size())));
-    std::vector<int2> tiles(o.ntiles + 1);
-    for (int64_t t = 0; t <= o.ntiles; ++t) tiles[t] = make_int2(hh.tiles[2 * t], hh.tiles[2 * t + 1]);
-    GBK_TRY(upload(&o.tiles, tiles));
-    GBK_TRY(upload(&o.seg_tile, hh.seg_tile));
+    const int H = hot_slots(vs);
+    o.H = H;
+    {
+        TmpBuf deg, ckey, ckey_out, corder, ciota, slot;
+        GBK_TRY(deg.alloc(4 * (n + 1)));
+        GBK_TRY(ckey.alloc(4 * (n > 0 ? n : 1)));
+        GBK_TRY(ckey_out.alloc(4 * (n > 0 ? n : 1)));
+        GBK_TRY(corder.alloc(4 * (n > 0 ? n : 1)));
+        GBK_TRY(ciota.alloc(4 * (n > 0 ? n : 1)));
+        GBK_TRY(slot.alloc(4 * (n > 0 ? n : 1)));
+        GBK_HIP(hipMemsetAsync(deg.p, 0, 4 * (n + 1), st));
+        GBK_TRY(dev_count(m.col, nnz, deg.as<int32_t>(), st));
+        const int segbits = bits_for(nseg - 1);
+        const int db = 32 - segbits;
+        const size_t nh = (size_t)nseg * (H > 0 ? H : 1);
+        GBK_TRY(dev_alloc((void**)&o.hot_ids, 4 * nh));
+        GBK_TRY(dev_alloc(&o.xh, vs * nh));
+        hipLaunchKernelGGL(k_fill_i32, grid_e((int64_t)nh), dim3(256), 0, st, o.hot_ids, (int64_t)nh, -1);
+        GBK_LAUNCH_CHECK();
+        if (n > 0) {
+            hipLaunchKernelGGL(k_fill_i32, grid_e(n), dim3(256), 0, st, slot.as<int32_t>(), n, -1);
+            GBK_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_hot_key, grid_e(n), dim3(256), 0, st, deg.as<int32_t>(), n, w, db,
+                               ckey.as<uint32_t>());
+            GBK_LAUNCH_CHECK();
+            GBK_TRY(dev_iota(ciota.as<int32_t>(), n, st));
+            GBK_TRY(dev_sort_pairs(ckey.as<uint32_t>(), ckey_out.as<uint32_t>(), ciota.as<int32_t>(),
+                                   corder.as<int32_t>(), n, 32, st));
+            if (H > 0) {
+                hipLaunchKernelGGL(k_hot_assign, grid_e(n), dim3(256), 0, st, corder.as<int32_t>(),
+                                   deg.as<int32_t>(), n, w, H, slot.as<int32_t>(), o.hot_ids);
+                GBK_LAUNCH_CHECK();
+            }
+        }
+        GBK_TRY(dev_alloc((void**)&o.hcol, 4 * z1));
+        if (nnz > 0) {
+            hipLaunchKernelGGL(k_hot_encode, grid_e(nnz), dim3(256), 0, st, s.v.col, slot.as<int32_t>(), nnz,
+                               o.hcol);
+            GBK_LAUNCH_CHECK();
+        }
+        GBK_HIP(hipStreamSynchronize(st));
+    }
+    // wave tiles per segment
+    std::vector<int32_t> seg_k(nseg + 1);
+    {
+        TmpBuf sp_d, sk_d;
+        GBK_TRY(sp_d.alloc(4 * (nseg + 1)));
+        GBK_TRY(sk_d.alloc(4 * (nseg + 1)));
+        GBK_HIP(hipMemcpyAsync(sp_d.p, seg_pair.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, st));
+        GBK_TRY(dev_gather_i32(s.v.row_ptr, sp_d.as<int32_t>(), nseg + 1, sk_d.as<int32_t>(), st));
+        GBK_HIP(hipMemcpyAsync(seg_k.data(), sk_d.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, st));
+        GBK_HIP(hipStreamSynchronize(st));
+    }
+    std::vector<int64_t> seg_tile(nseg + 1, 0);
+    for (int64_t g = 0; g < nseg; ++g) {
+        const int64_t items = (int64_t)(seg_pair[g + 1] - seg_pair[g]) + (int64_t)(seg_k[g + 1] - seg_k[g]);
+        seg_tile[g + 1] = seg_tile[g] + (items + HOT_WT - 1) / HOT_WT;
+    }
+    o.ntiles = seg_tile[nseg];
+    GBK_TRY(upload(&o.seg_tile, seg_tile));
+    GBK_TRY(upload(&o.seg_pair, seg_pair));
+    GBK_TRY(dev_alloc((void**)&o.tiles, sizeof(int2) * (o.ntiles + 1)));
+    hipLaunchKernelGGL(k_seg_tiles, grid_e(o.ntiles + 1), dim3(256), 0, st, s.v.row_ptr, o.seg_tile, o.seg_pair,
+                       (int)nseg, o.ntiles, npairs, nnz, o.tiles);
+    GBK_LAUNCH_CHECK();
     GBK_TRY(dev_alloc(&o.carry, 2 * vs * (size_t)(o.ntiles > 0 ? o.ntiles : 1)));
+    GBK_HIP(hipStreamSynchronize(st));
     return GBK_SUCCESS;
 }
 
 static int build_seg(const DevCSR& m, int type, SegCSR** out) {
-    std::vector<int32_t> rp, col;
-    std::vector<char> val;
-    GBK_TRY(download_csr(m, type, rp, col, val));
     const int64_t n = m.nrows;
+    if (m.nnz >= (int64_t)INT32_MAX / 2) return fail(GBK_INVALID_VALUE, "too many entries for the segmented format");
     // a multiple of 8 segments (one group per XCD), each at most 512 Ki
     // columns wide
     const int64_t sc = seg_cols_default(n);
     const int64_t nseg = 8 * std::max<int64_t>(1, (n + 8 * sc - 1) / (8 * sc));
     const int64_t w = std::max<int64_t>(1, (n + nseg - 1) / nseg);
-    std::vector<int64_t> bounds;
-    for (int64_t i = 0; i <= nseg; ++i) bounds.push_back(std::min(n, i * w));
-    gbk_host::SegHost h;
-    gbk_host::build_segmented(n, rp.data(), col.data(), val.data(), type_size(type), bounds, nullptr, h);
     SegCSR* s = new SegCSR();
-    int rc = build_seg_common(h, type, n, *s);
-    if (rc == GBK_SUCCESS) rc = build_seg_hot_dev(h, type, n, *s);
+    const int rc = build_seg_device(m, type, nseg, w, *s);
     if (rc != GBK_SUCCESS) {
         free_seg_fields(*s);
         delete s;

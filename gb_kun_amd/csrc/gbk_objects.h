@@ -154,6 +154,17 @@ int spmv_masked_rows(gbk_matrix A, int transposed, const void* x, void* y, int a
                      const int32_t* rows, int64_t nrows_list);
 int build_merge_tiles(DevCSR& m, int type);
 
+// Device construction steps (device_build.hip).
+int bits_for(int64_t maxval);
+int dev_sort_pairs(const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t m, int bits,
+                   hipStream_t st);
+int dev_iota(int32_t* v, int64_t n, hipStream_t st);
+int dev_row_ids(const int32_t* rp, int64_t n, int32_t* rowid, hipStream_t st);
+int dev_count(const int32_t* key, int64_t m, int32_t* cnt, hipStream_t st);
+int dev_gather_i32(const int32_t* src, const int32_t* idx, int64_t m, int32_t* out, hipStream_t st);
+int dev_gather_bytes(const void* src, const int32_t* idx, int64_t m, size_t vs, void* out, hipStream_t st);
+int device_transpose(const DevCSR& in, int type, int64_t n, DevCSR& out);
+
 // spmspv.hip
 int spmspv(gbk_matrix A, bool use_csr, const int32_t* x_ids, const void* x_vals, int64_t x_nvals,
            void* y, int add, int mul, int accum, gbk_vector mask, int mask_mode);

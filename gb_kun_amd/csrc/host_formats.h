@@ -34,19 +34,6 @@ struct SegHost {
 
 void build_sell_c(int64_t n, const int32_t* rp, const int32_t* col, const void* val, size_t vs, int C, int sigma,
                   int64_t long_thr, SellCHost& out);
-// Hot columns of the segmented format: per segment, the (at most) H columns
-// of highest in-degree (at least 2) get an LDS slot; their entries are
-// re-encoded as -1 - slot.  Tiles of `wave_items` merge items cover each
-// segment's (segment, row) pairs without crossing a segment.
-struct SegHotHost {
-    int H = 0;
-    std::vector<int32_t> hcol;      // entries of SegHost::col, hot ones as -1 - slot
-    std::vector<int32_t> hot_ids;   // nseg * H original columns, -1 = unused slot
-    std::vector<int32_t> tiles;     // 2 * (ntiles + 1): (virtual row, entry) of each tile start
-    std::vector<int64_t> seg_tile;  // nseg + 1: first tile of each segment
-};
-void build_seg_hot(const SegHost& h, int64_t ncols, int H, int64_t wave_items, SegHotHost& out);
-
 void build_segmented(int64_t n, const int32_t* rp, const int32_t* col, const void* val, size_t vs,
                      const std::vector<int64_t>& seg_bounds, const int32_t* col_relabel, SegHost& out);
 void lav_relabel(int64_t n, const int32_t* rp, const int32_t* col, double dense_frac, int64_t seg_cols,

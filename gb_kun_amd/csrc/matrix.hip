@@ -66,12 +66,9 @@ static int build_common(gbk_matrix A, int64_t n, const int32_t* rp, const int32_
     A->n = n;
     A->nnz = nnz;
     GBK_TRY(upload_csr(A->csr, A->type, n, rp, col, val));
-    {
-        std::vector<int32_t> trp(n + 1), tcol(nnz > 0 ? nnz : 1);
-        std::vector<char> tval(vs * (nnz > 0 ? nnz : 1));
-        gbk_host::transpose_csr(n, rp, col, val, vs, trp.data(), tcol.data(), tval.data());
-        GBK_TRY(upload_csr(A->csc, A->type, n, trp.data(), tcol.data(), tval.data()));
-    }
+    // CSC by a stable sort of the entries by column on the device
+    GBK_TRY(device_transpose(A->csr, A->type, n, A->csc));
+    (void)vs;
     for (auto& b : A->ws.buf) GBK_TRY(dev_alloc(&b, sizeof(double) * (n > 0 ? n : 1)));
     A->ws.bytes = 4 * sizeof(double) * (n > 0 ? n : 1);
     A->built = true;
