@@ -648,6 +648,30 @@ __global__ void k_seg_merge(const int32_t* __restrict__ rpp, const int32_t* __re
     y[r] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[r], acc);
 }
 
+// The same merge when a row has at most 8 pairs (8 segments): the pair ids
+// and then the sums are loaded unconditionally (clamped to the row's last
+// pair), so each row costs two rounds of loads instead of a chain.
+template <typename T, int ADD>
+__global__ void k_seg_merge8(const int32_t* __restrict__ rpp, const int32_t* __restrict__ row_pairs,
+                             const T* __restrict__ partial, T* __restrict__ y, int64_t n, int accum) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int p0 = rpp[r], cnt = rpp[r + 1] - p0;
+    T acc = add_identity<ADD, T>();
+    if (cnt > 0) {
+        int pid[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pid[k] = row_pairs[p0 + min(k, cnt - 1)];
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[pid[k]];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < cnt) acc = add_op<ADD, T>(acc, v[k]);
+    }
+    y[r] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[r], acc);
+}
+
 template <typename T> __device__ __forceinline__ T hot_shfl_up(T v, int off) {
     if constexpr (sizeof(T) >= 4) return __shfl_up(v, off, 64);
     else return (T)__shfl_up((int)v, off, 64);
@@ -861,8 +885,12 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
         GBK_LAUNCH_CHECK();
     }
     if (s->n > 0) {
-        hipLaunchKernelGGL((k_seg_merge<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st, s->rpp,
-                           s->row_pairs, (const T*)s->partial, y, s->n, accum);
+        if (s->nseg <= 8)
+            hipLaunchKernelGGL((k_seg_merge8<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st,
+                               s->rpp, s->row_pairs, (const T*)s->partial, y, s->n, accum);
+        else
+            hipLaunchKernelGGL((k_seg_merge<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st,
+                               s->rpp, s->row_pairs, (const T*)s->partial, y, s->n, accum);
         GBK_LAUNCH_CHECK();
     }
     return GBK_SUCCESS;
