@@ -66,7 +66,9 @@ def _run(cmd):
     return "\n".join(l for l in r.stdout.splitlines() if _HOST_NOISE not in l and l.strip())
 
 
-def build(verbose=True, jobs=None):
+def build(verbose=True, jobs=None, force=False):
+    """force=True recompiles every source even when an object of the same
+    content is cached in _build/."""
     os.makedirs(BUILD, exist_ok=True)
     hip, cpp = _sources()
     hdr = _headers_digest()
@@ -75,12 +77,12 @@ def build(verbose=True, jobs=None):
     for s in hip:
         o = _obj_path(s, HIP_FLAGS, hdr)
         objs.append(o)
-        if not os.path.exists(o):
+        if force or not os.path.exists(o):
             jobs_list.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
     for s in cpp:
         o = _obj_path(s, CXX_FLAGS, hdr)
         objs.append(o)
-        if not os.path.exists(o):
+        if force or not os.path.exists(o):
             jobs_list.append(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
     if jobs_list:
         n = jobs or min(8, os.cpu_count() or 4)
