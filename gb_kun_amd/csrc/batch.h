@@ -31,8 +31,9 @@ int exclusive_scan_i32_on(const int32_t* in, int32_t* out, int64_t n, int32_t* t
 // SpMM tiles are built on first use and kept with the matrix orientation.
 int ensure_spmm_tiles(DevCSR& m, hipStream_t st);
 size_t spmm_carry_bytes(const DevCSR& m, int type, int k);
+// k columns of X and Y, rows ld apart (ld = 0: k).
 int spmm_run(DevCSR& m, int type, const void* X, void* Y, int k, int add, int mul, int accum, void* carry,
-             hipStream_t st);
+             hipStream_t st, int ld = 0);
 
 size_t merge_carry_bytes(const DevCSR& m);
 int spmv_merge_ws(const DevCSR& m, int type, const void* x, void* y, int add, int mul, int accum, void* carry);

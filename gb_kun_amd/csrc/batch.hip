@@ -77,8 +77,9 @@ template <typename T> __device__ __forceinline__ T shfl_t(T v, int src, int widt
 template <typename T, int ADD, int MUL, int LW>
 __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ X, T* __restrict__ Y,
-                                              int32_t nrows, int K, const int2* __restrict__ tiles, int64_t ntiles,
-                                              T* __restrict__ carry_head, T* __restrict__ carry_tail, int accum) {
+                                              int32_t nrows, int K, int ldx, const int2* __restrict__ tiles,
+                                              int64_t ntiles, T* __restrict__ carry_head, T* __restrict__ carry_tail,
+                                              int accum) {
     constexpr int GPB = 256 / LW;
     // Row ends of the block's rows, staged in LDS by the whole block (eight
     // loads in flight per thread), so that the walk below reads them there
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, co
         if (r == c0.x && head_open) {
             carry_head[t * K + lane] = v;
         } else {
-            T* py = Y + (int64_t)r * K + lane;
+            T* py = Y + (int64_t)r * ldx + lane;
             *py = (accum == GBK_ACC_NONE) ? v : binop<T>(accum, *py, v);
         }
     };
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, co
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int ci = __shfl(cc, i, LW);
-            xv[i] = (i < nb && act) ? X[(int64_t)ci * K + lane] : T(0);
+            xv[i] = (i < nb && act) ? X[(int64_t)ci * ldx + lane] : T(0);
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void k_spmm(const int32_t* __restrict__ rp, co
 // Adds the carried partials of every row that begins before its ending tile.
 template <typename T, int ADD, int LW>
 __global__ __launch_bounds__(256) void k_spmm_fixup(const int32_t* __restrict__ rp, const int2* __restrict__ tiles,
-                                                    int64_t ntiles, int K, const T* __restrict__ carry_head,
+                                                    int64_t ntiles, int K, int ldx, const T* __restrict__ carry_head,
                                                     const T* __restrict__ carry_tail, T* __restrict__ Y, int accum) {
     constexpr int GPB = 256 / LW;
     const int64_t t = (int64_t)blockIdx.x * GPB + threadIdx.x / LW;
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_spmm_fixup(const int32_t* __restrict__ 
         total = add_op<ADD, T>(carry_tail[u * K + lane], total);
         if (tiles[u].y <= start) break;
     }
-    T* py = Y + (int64_t)r0 * K + lane;
+    T* py = Y + (int64_t)r0 * ldx + lane;
     *py = (accum == GBK_ACC_NONE) ? total : binop<T>(accum, *py, total);
 }
 
@@ -193,8 +194,10 @@ size_t spmm_carry_bytes(const DevCSR& m, int type, int k) {
 }
 
 int spmm_run(DevCSR& m, int type, const void* X, void* Y, int k, int add, int mul, int accum, void* carry,
-             hipStream_t st) {
+             hipStream_t st, int ld) {
     if (k < 1 || k > 64) return fail(GBK_INVALID_VALUE, "SpMM needs 1 <= k <= 64 right-hand sides (got %d)", k);
+    if (ld == 0) ld = k;
+    if (ld < k) return fail(GBK_INVALID_VALUE, "SpMM row stride %d below its %d columns", ld, k);
     if (order_sensitive_add(add))
         return fail(GBK_INVALID_VALUE, "SpMM needs a commutative monoid (got add=%d)", add);
     GBK_TRY(ensure_spmm_tiles(m, st));
@@ -213,11 +216,11 @@ int spmm_run(DevCSR& m, int type, const void* X, void* Y, int k, int add, int mu
     {                                                                                                           \
         constexpr int GPB = 256 / LWV;                                                                          \
         hipLaunchKernelGGL((k_spmm<T, ADD, MUL, LWV>), grid_of(nt, GPB), dim3(256), 0, st, m.row_ptr, m.col,     \
-                           (const T*)m.val, (const T*)X, (T*)Y, (int32_t)m.nrows, k, m.spmm_tiles, nt, head,     \
-                           tail, accum);                                                                        \
+                           (const T*)m.val, (const T*)X, (T*)Y, (int32_t)m.nrows, k, ld, m.spmm_tiles, nt,      \
+                           head, tail, accum);                                                                  \
         GBK_LAUNCH_CHECK();                                                                                     \
         hipLaunchKernelGGL((k_spmm_fixup<T, ADD, LWV>), grid_of(nt, GPB), dim3(256), 0, st, m.row_ptr,           \
-                           m.spmm_tiles, nt, k, (const T*)head, (const T*)tail, (T*)Y, accum);                  \
+                           m.spmm_tiles, nt, k, ld, (const T*)head, (const T*)tail, (T*)Y, accum);              \
         GBK_LAUNCH_CHECK();                                                                                     \
         return GBK_SUCCESS;                                                                                     \
     }

@@ -58,8 +58,9 @@ __global__ void k_compact_flags(const int32_t* __restrict__ flag, const int32_t*
 
 // sink[q] = sum over sinks of R[v][q] (fp64 accumulation, one atomic per
 // wave-column pair).  Block = 256 threads = 256/k sinks x k columns.
-__global__ __launch_bounds__(256) void k_ppr_sink(const float* __restrict__ R, int k, const int32_t* __restrict__ sinks,
-                                                  int64_t nsinks, double* __restrict__ sink) {
+__global__ __launch_bounds__(256) void k_ppr_sink(const float* __restrict__ R, int k, int ld,
+                                                  const int32_t* __restrict__ sinks, int64_t nsinks,
+                                                  double* __restrict__ sink) {
     __shared__ double s[256];
     const int per = 256 / k;
     const int q = threadIdx.x % k;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(256) void k_ppr_sink(const float* __restrict__ R, i
     double acc = 0.0;
     if (j < per) {
         for (int64_t i = (int64_t)blockIdx.x * per + j; i < nsinks; i += (int64_t)gridDim.x * per)
-            acc += (double)R[(int64_t)sinks[i] * k + q];
+            acc += (double)R[(int64_t)sinks[i] * ld + q];
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(256) void k_ppr_sink(const float* __restrict__ R, i
 // L1 change of every column to delta[q].
 // Rn may be Y (each element is read, then written, by the same thread).
 __global__ __launch_bounds__(256) void k_ppr_update(const float* __restrict__ R, const float* Y, float* Rn, int64_t n,
-                                                    int k,
+                                                    int k, int ld,
                                                     const int32_t* __restrict__ src, const double* __restrict__ sink,
                                                     const int32_t* __restrict__ done, float damping,
                                                     double* __restrict__ delta) {
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(256) void k_ppr_update(const float* __restrict__ R,
         const int sq = src[q];
         const bool frozen = done[q] != 0;
         for (int64_t v = (int64_t)blockIdx.x * per + j; v < n; v += (int64_t)gridDim.x * per) {
-            const int64_t i = v * k + q;
+            const int64_t i = v * ld + q;
             const float old = R[i];
             float nv;
             if (frozen) {
@@ -219,10 +220,10 @@ int ppr_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_by
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, L.sink, 64);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, L.delta, 64);
         if (nsinks > 0 && sb > 0)
-            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st, R, k, L.sinks, (int64_t)nsinks,
+            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st, R, k, k, L.sinks, (int64_t)nsinks,
                                L.sink);
         // Y is overwritten with the new ranks, then R and Y swap roles
-        hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)std::max<int64_t>(ub, 1)), dim3(256), 0, st, R, Y, Y, n, k,
+        hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)std::max<int64_t>(ub, 1)), dim3(256), 0, st, R, Y, Y, n, k, k,
                            L.src, L.sink, L.done, (float)damping, L.delta);
         if (hipGetLastError() != hipSuccess) {
             rc = fail(GBK_PANIC, "PageRank kernel launch failed");
@@ -279,18 +280,23 @@ __global__ void k_sssp_reset_cols(int32_t* __restrict__ D, int64_t n, int k, uin
     D[i] = (src[q] == v) ? 0 : INT32_MAX;
 }
 
-// D = min(D, Y); bit q of *changed is set if column q improved anywhere.
+// D = min(D, Y) over the first kw of the ld columns of n rows; bit q of
+// *changed is set if column q improved anywhere.
 __global__ __launch_bounds__(256) void k_sssp_min_update(int32_t* __restrict__ D, const int32_t* __restrict__ Y,
-                                                         int64_t total, int k, unsigned long long* __restrict__ changed) {
+                                                         int64_t n, int kw, int ld,
+                                                         unsigned long long* __restrict__ changed) {
     __shared__ unsigned long long s_m;
     if (threadIdx.x == 0) s_m = 0;
     __syncthreads();
     unsigned long long m = 0;
+    const int64_t total = n * kw;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t y = Y[i];
-        if (y < D[i]) {
-            D[i] = y;
-            m |= 1ull << (int)(i % k);
+        const int q = (int)(i % kw);
+        const int64_t e = (i / kw) * ld + q;
+        const int32_t y = Y[e];
+        if (y < D[e]) {
+            D[e] = y;
+            m |= 1ull << q;
         }
     }
 #pragma unroll
@@ -338,12 +344,12 @@ static dim3 capped_b(int64_t threads) {
 
 // One round: Y = A (min.+) D, D = min(D, Y); *changed_out receives the mask of
 // columns that changed.
-static int mssp_round(gbk_matrix A, const MsspLayout& L, int k, hipStream_t st, unsigned long long* hmask,
+static int mssp_round(gbk_matrix A, const MsspLayout& L, int kw, int ld, hipStream_t st, unsigned long long* hmask,
                       uint64_t* changed_out) {
     const int64_t n = A->n;
-    GBK_TRY(spmm_run(A->csr, GBK_INT32, L.D, L.Y, k, GBK_ADD_MIN, GBK_MUL_PLUS, GBK_ACC_NONE, L.carry, st));
+    GBK_TRY(spmm_run(A->csr, GBK_INT32, L.D, L.Y, kw, GBK_ADD_MIN, GBK_MUL_PLUS, GBK_ACC_NONE, L.carry, st, ld));
     hipLaunchKernelGGL(k_zero_mask, dim3(1), dim3(64), 0, st, L.changed);
-    hipLaunchKernelGGL(k_sssp_min_update, capped_b(n * k), dim3(256), 0, st, L.D, L.Y, n * k, k, L.changed);
+    hipLaunchKernelGGL(k_sssp_min_update, capped_b(n * kw), dim3(256), 0, st, L.D, L.Y, n, kw, ld, L.changed);
     GBK_LAUNCH_CHECK();
     GBK_HIP(hipMemcpyAsync(hmask, L.changed, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     GBK_HIP(hipStreamSynchronize(st));
@@ -369,7 +375,7 @@ int mssp_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_b
     int64_t rounds = 0;
     uint64_t changed = all;
     while (changed) {
-        GBK_TRY(mssp_round(A, L, k, st, &hmask, &changed));
+        GBK_TRY(mssp_round(A, L, k, k, st, &hmask, &changed));
         ++rounds;
     }
     hipLaunchKernelGGL(k_transpose_nk<int32_t>, grid_b(n, 64), dim3(256), 0, st, L.D, n, k, dist_out);
@@ -472,29 +478,34 @@ class PprEngine : public Engine {
             GBK_HIP(hipMemcpyAsync(L_.done, done_, sizeof(int32_t) * k_, hipMemcpyHostToDevice, st_));
             GBK_HIP(hipMemcpyAsync(L_.src, src_, sizeof(int32_t) * k_, hipMemcpyHostToDevice, st_));
         }
+        // Only the columns up to the highest occupied slot are advanced (the
+        // server fills the lowest free slot first): a lightly loaded engine
+        // runs a narrow SpMM.  Columns beyond it are reseeded in full when a
+        // query is admitted there.
+        const int kw = 64 - __builtin_clzll(occupied_);
         hipStream_t prev = stream();
         set_thread_stream(st_);
         int rc;
         if (k_ == 1) rc = spmv_merge_ws(A_->csr, GBK_FP32, L_.R, L_.Y, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L_.carry);
-        else rc = spmm_run(A_->csr, GBK_FP32, L_.R, L_.Y, k_, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L_.carry, st_);
+        else rc = spmm_run(A_->csr, GBK_FP32, L_.R, L_.Y, kw, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L_.carry, st_, k_);
         set_thread_stream(prev);
         GBK_TRY(rc);
-        const int per = 256 / k_;
+        const int per = 256 / kw;
         const int64_t ub = std::max<int64_t>(std::min<int64_t>((n_ + per - 1) / per, (int64_t)device_cu_count() * 32), 1);
         const int64_t sb = std::min<int64_t>(((int64_t)nsinks_ + per - 1) / per, (int64_t)device_cu_count() * 8);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink, 64);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.delta, 64);
         if (nsinks_ > 0 && sb > 0)
-            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st_, L_.R, k_, L_.sinks, (int64_t)nsinks_,
-                               L_.sink);
-        hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)ub), dim3(256), 0, st_, L_.R, L_.Y, L_.Y, n_, k_, L_.src, L_.sink,
-                           L_.done, (float)damping_, L_.delta);
+            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st_, L_.R, kw, k_, L_.sinks,
+                               (int64_t)nsinks_, L_.sink);
+        hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)ub), dim3(256), 0, st_, L_.R, L_.Y, L_.Y, n_, kw, k_, L_.src,
+                           L_.sink, L_.done, (float)damping_, L_.delta);
         GBK_LAUNCH_CHECK();
-        GBK_HIP(hipMemcpyAsync(hdelta_, L_.delta, sizeof(double) * k_, hipMemcpyDeviceToHost, st_));
+        GBK_HIP(hipMemcpyAsync(hdelta_, L_.delta, sizeof(double) * kw, hipMemcpyDeviceToHost, st_));
         GBK_HIP(hipStreamSynchronize(st_));
         std::swap(L_.R, L_.Y);
         bool changed = false;
-        for (int q = 0; q < k_; ++q) {
+        for (int q = 0; q < kw; ++q) {
             if (!((occupied_ >> q) & 1ull) || done_[q]) continue;
             ++iters_[q];
             if ((tol_ > 0.0 && hdelta_[q] < tol_) || iters_[q] >= itermax_) {
@@ -584,9 +595,11 @@ class SsspEngine : public Engine {
             pend_slot_.clear();
         }
         if (!occupied_) return GBK_SUCCESS;
+        // only the columns up to the highest occupied slot take part
+        const int kw = 64 - __builtin_clzll(occupied_);
         uint64_t changed = 0;
-        GBK_TRY(mssp_round(A_, L_, k_, st_, hmask_, &changed));
-        for (int q = 0; q < k_; ++q)
+        GBK_TRY(mssp_round(A_, L_, kw, k_, st_, hmask_, &changed));
+        for (int q = 0; q < kw; ++q)
             if (((occupied_ >> q) & 1ull) && !((changed >> q) & 1ull)) finished.push_back(q);
         return GBK_SUCCESS;
     }
