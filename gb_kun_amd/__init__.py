@@ -87,6 +87,8 @@ _sig("gbk_matrix_set_format", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
 _sig("gbk_matrix_get_format", C.c_int, _vp, C.POINTER(C.c_int))
 _sig("gbk_matrix_build_coo", C.c_int, _vp, _pi64, _pi64, _vp, _i64)
 _sig("gbk_matrix_read_mtx", C.c_int, _vp, C.c_char_p)
+_sig("gbk_mxm", C.c_int, _vp, _vp, C.c_int, Semiring, _vp, _vp, C.POINTER(Desc))
+_sig("gbk_tricount", C.c_int, _vp, C.c_int, C.POINTER(C.c_uint64))
 _sig("gbk_mtx_read", C.c_int, C.c_char_p, _pi64, _pi64, _pi64, C.POINTER(_pi64), C.POINTER(_pi64))
 _sig("gbk_mtx_write", C.c_int, C.c_char_p, _i64, _i64, _pi64, _pi64, _i64)
 _sig("gbk_matrix_build_csr", C.c_int, _vp, _pi64, _pi64, _vp, _i64, _i64)
@@ -452,6 +454,20 @@ def vxm(w, mask, accum, semiring, u, A, desc_mask=DEFAULT):
     add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
     check(lib.gbk_vxm(w.h, mask.h if mask is not None else None, accum, Semiring(add, mul), u.h, A.h,
                       _desc(desc_mask)))
+
+
+def mxm(Cm, mask, accum, semiring, A, B, desc_mask=DEFAULT):
+    """C<mask> = A (+.*) B (structural mask; C gets the mask's entries)."""
+    add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
+    check(lib.gbk_mxm(Cm.h, mask.h if mask is not None else None, accum, Semiring(add, mul), A.h, B.h,
+                      _desc(desc_mask)))
+
+
+def tricount(A, method=0):
+    """Triangle count of a symmetric matrix (LAGr_TriangleCount methods 0-6)."""
+    v = C.c_uint64(0)
+    check(lib.gbk_tricount(A.h, method, C.byref(v)))
+    return v.value
 
 
 def spmv_device(A, x_ptr, y_ptr, semiring=(ADD_PLUS, MUL_TIMES), accum=ACC_NONE, transposed=False):

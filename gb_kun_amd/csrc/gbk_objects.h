@@ -164,6 +164,7 @@ int dev_count(const int32_t* key, int64_t m, int32_t* cnt, hipStream_t st);
 int dev_gather_i32(const int32_t* src, const int32_t* idx, int64_t m, int32_t* out, hipStream_t st);
 int dev_gather_bytes(const void* src, const int32_t* idx, int64_t m, size_t vs, void* out, hipStream_t st);
 int device_transpose(const DevCSR& in, int type, int64_t n, DevCSR& out);
+int build_from_device_csr(gbk_matrix A, int64_t n, int64_t nnz, int32_t* rp, int32_t* col, void* val);
 
 // spmspv.hip
 int spmspv(gbk_matrix A, bool use_csr, const int32_t* x_ids, const void* x_vals, int64_t x_nvals,

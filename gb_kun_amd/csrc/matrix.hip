@@ -76,6 +76,53 @@ static int build_common(gbk_matrix A, int64_t n, const int32_t* rp, const int32_
     return GBK_SUCCESS;
 }
 
+__global__ void k_row_len_max(const int32_t* __restrict__ rp, int64_t n, int32_t* __restrict__ mx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t v = 0;
+    if (i < n) v = rp[i + 1] - rp[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(mx, v);
+}
+
+// Builds A from an n x n CSR already on the device; A takes ownership of
+// the three arrays (released by gbk_matrix_free or the next build).
+int build_from_device_csr(gbk_matrix A, int64_t n, int64_t nnz, int32_t* rp, int32_t* col, void* val) {
+    GBK_TRY(ensure_init());
+    free_matrix_storage(A);
+    A->n = n;
+    A->nnz = nnz;
+    DevCSR& m = A->csr;
+    m.nrows = m.ncols = n;
+    m.nnz = nnz;
+    m.row_ptr = rp;
+    m.col = col;
+    m.val = val;
+    hipStream_t st = stream();
+    int32_t* mx = nullptr;
+    GBK_TRY(dev_alloc((void**)&mx, sizeof(int32_t)));
+    int32_t mxh = 0;
+    int rc = GBK_SUCCESS;
+    if (hipMemsetAsync(mx, 0, sizeof(int32_t), st) != hipSuccess) rc = fail(GBK_PANIC, "hipMemsetAsync failed");
+    if (rc == GBK_SUCCESS && n > 0) {
+        hipLaunchKernelGGL(k_row_len_max, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rp, n, mx);
+        if (hipGetLastError() != hipSuccess) rc = fail(GBK_PANIC, "kernel launch failed");
+    }
+    if (rc == GBK_SUCCESS && hipMemcpyAsync(&mxh, mx, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = fail(GBK_PANIC, "hipMemcpyAsync failed");
+    if (rc == GBK_SUCCESS && hipStreamSynchronize(st) != hipSuccess) rc = fail(GBK_PANIC, "sync failed");
+    dev_free(mx);
+    GBK_TRY(rc);
+    m.max_row = mxh;
+    GBK_TRY(build_merge_tiles(m, A->type));
+    GBK_TRY(device_transpose(A->csr, A->type, n, A->csc));
+    for (auto& b : A->ws.buf) GBK_TRY(dev_alloc(&b, sizeof(double) * (n > 0 ? n : 1)));
+    A->ws.bytes = 4 * sizeof(double) * (n > 0 ? n : 1);
+    A->built = true;
+    if (A->format != GBK_FMT_CSR) GBK_TRY(format_build(A));
+    return GBK_SUCCESS;
+}
+
 static int check_nnz(int64_t nnz) {
     if (nnz < 0) return fail(GBK_INVALID_VALUE, "negative nnz");
     if (nnz >= (int64_t)INT32_MAX) return fail(GBK_INVALID_VALUE, "nnz = %lld does not fit the int32 index layout", (long long)nnz);

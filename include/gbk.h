@@ -351,6 +351,24 @@ int gbk_sssp(gbk_matrix A, int64_t source, void* dist, int64_t* stats);
 /* Same, writing the distances (n entries of the matrix type) to a device array. */
 int gbk_sssp_device(gbk_matrix A, int64_t source, void* dist_dev, int64_t* stats);
 
+/* C<M> = A (+.*) B with a structural mask, replaces lablas::mxm with a mask
+ * (src/cpp_graphblas/operations.hpp:411-437; backend::mxm, operations.h:
+ * 436-486, masked IKJ: spmspm_masked_hash_based_gustavson.h:9-170).  C gets
+ * exactly the entries of M; C(i,j) = fold over k of A(i,k) (*) B(k,j), 0 where
+ * no k contributes.  A, B, M are n x n; A and B of one type, which C takes.
+ * accum is ignored (the reference rebuilds C); a complemented mask or a NULL
+ * mask is an error. */
+int gbk_mxm(gbk_matrix C, gbk_matrix M, int accum, gbk_semiring op, gbk_matrix A, gbk_matrix B,
+            const gbk_desc* desc);
+
+/* Triangle count of a symmetric matrix, replaces LAGr_TriangleCount
+ * (algorithms/tc/tc.hpp:80-291).  method: 0 default (= 1), 1 Burkhardt
+ * sum((A^2) .* A) / 6, 2 Cohen sum((L U) .* A) / 2, 3 Sandia sum((L L) .* L),
+ * 4 Sandia2 sum((U U) .* U), 5 SandiaDot sum((L U') .* L), 6 SandiaDot2
+ * sum((U L') .* U), L = tril(A, -1), U = triu(A, 1).  All give the same count
+ * (the reference divides the Cohen sum by 6). */
+int gbk_tricount(gbk_matrix A, int method, uint64_t* ntriangles);
+
 /* Connected components, replaces lablas::algorithm::cc (FastSV,
  * algorithms/cc/cc.hpp:12-119).  labels (host int32): the minimum vertex id
  * of each component, the fixed point FastSV converges to. A must be

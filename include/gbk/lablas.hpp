@@ -372,6 +372,24 @@ LA_Info vxm(Vector<W>* w, const Vector<M>* mask, NULL_TYPE, SemiringT op, const 
     return vxm(w, mask, second<U, W, U>(), op, u, A, desc);
 }
 
+// mxm with a mask (operations.hpp:411-437): C gets the mask's entries.
+template <typename c, typename m, typename a, typename b, typename BinaryOpTAccum, typename SemiringT>
+LA_Info mxm(Matrix<c>* C, const Matrix<m>* mask, BinaryOpTAccum, SemiringT, const Matrix<a>* A, const Matrix<b>* B,
+            Descriptor* desc) {
+    if (C == nullptr || A == nullptr || B == nullptr || desc == nullptr) return GrB_UNINITIALIZED_OBJECT;
+    gbk_desc d = desc->c_desc();
+    const int rc = gbk_mxm(C->handle(), mask ? mask->handle() : nullptr, detail::accum_id<BinaryOpTAccum>(),
+                           detail::sr<SemiringT>(), A->handle(), B->handle(), &d);
+    if (rc != GBK_SUCCESS && (d.mask == GBK_COMP || d.mask == GBK_STR_COMP))
+        throw "Error: complementary mask is not supported yet";
+    return detail::code(rc);
+}
+template <typename c, typename m, typename a, typename b, typename SemiringT>
+LA_Info mxm(Matrix<c>* C, const Matrix<m>* mask, NULL_TYPE, SemiringT op, const Matrix<a>* A, const Matrix<b>* B,
+            Descriptor* desc) {
+    return mxm(C, mask, second<c, a, b>(), op, A, B, desc);
+}
+
 template <typename W, typename M, typename U, typename V, typename BinaryOpTAccum, typename BinaryOpT>
 LA_Info eWiseAdd(Vector<W>* w, const Vector<M>* mask, BinaryOpTAccum, BinaryOpT, const Vector<U>* u, const Vector<V>* v,
                  Descriptor* desc) {
@@ -539,6 +557,13 @@ inline void page_rank_sinks(Vector<float>* centrality, const Matrix<float>* A, i
     centrality->build(r.data(), n);
 }
 
+// LAGr_TriangleCount (algorithms/tc/tc.hpp:80-291) on a symmetric matrix;
+// method numbering as LAGraph_TriangleCount_Method (0 = Burkhardt).
+template <typename T> LA_Info tricount(uint64_t* ntriangles, const Matrix<T>* A, int method = 0) {
+    if (ntriangles == nullptr || A == nullptr) return GrB_NULL_POINTER;
+    return detail::code(gbk_tricount(A->handle(), method, ntriangles));
+}
+
 }  // namespace algorithm
 }  // namespace lablas
 
@@ -547,3 +572,4 @@ inline void page_rank_sinks(Vector<float>* centrality, const Matrix<float>* A, i
 #define GrB_NULL ((long int)0)
 template <typename... Args> LA_Info GrB_mxv(Args... a) { return lablas::mxv(a...); }
 template <typename... Args> LA_Info GrB_vxm(Args... a) { return lablas::vxm(a...); }
+template <typename... Args> LA_Info GrB_mxm(Args... a) { return lablas::mxm(a...); }

@@ -23,6 +23,9 @@
  *   orc_pagerank_f64    golden PageRank of the compiled reference; fixed point
  *   orc_ppr_f64         closed-form linear solve; uniform teleport reduces it
  *                       to orc_pagerank_f64
+ *   orc_mxm_masked      dense products on small matrices (plus.times,
+ *                       min.plus), restricted to the mask's pattern
+ *   orc_masked_count    trace(A^3) / 6 of symmetric graphs (scipy)
  *
  * Conventions: CSR with int64 row_ptr[n+1] and int32 col[nnz]; values and
  * vectors are arrays of one element type given by `type`
@@ -600,3 +603,80 @@ int orc_ppr_f64(int64_t n, const int64_t* rp, const int32_t* col, const float* v
 }
 
 int orc_version(void) { return 1; }
+
+/* ------------------------------------------------------------------------ */
+/* Masked SpGEMM, SpMSpM_masked_ikj                                         */
+/* (src/backend/spmspm/spmspm_masked_hash_based_gustavson.h:9-170): per row  */
+/* i, the mask row's columns index a dense marker; for k in A(i,:) and j in */
+/* B(k,:) with j in the mask row, the product is folded into that slot;     */
+/* slots without a product hold 0 (the reference zero-fills its values).    */
+/* The reference stores a row's sums in hash-map order, so they can land on */
+/* other columns; here each sum stays at its own column.  A column repeated */
+/* in a mask row is matched at its first occurrence.                        */
+/* ------------------------------------------------------------------------ */
+#define ORC_MXM(SUF, T)                                                                                     \
+    static void mxm_##SUF(int64_t n, const int64_t* mrp, const int32_t* mcol, const int64_t* arp,          \
+                          const int32_t* acol, const T* aval, const int64_t* brp, const int32_t* bcol,      \
+                          const T* bval, int add, int mul, T* out) {                                        \
+        int64_t* pos = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));                        \
+        char* hit = (char*)calloc((size_t)(mrp[n] > 0 ? mrp[n] : 1), 1);                                   \
+        for (int64_t j = 0; j < n; ++j) pos[j] = -1;                                                        \
+        for (int64_t i = 0; i < n; ++i) {                                                                   \
+            for (int64_t p = mrp[i]; p < mrp[i + 1]; ++p)                                                   \
+                if (pos[mcol[p]] < 0) pos[mcol[p]] = p;                                                     \
+            for (int64_t e = arp[i]; e < arp[i + 1]; ++e) {                                                 \
+                const int64_t k = acol[e];                                                                  \
+                for (int64_t q = brp[k]; q < brp[k + 1]; ++q) {                                             \
+                    const int64_t p = pos[bcol[q]];                                                         \
+                    if (p < 0) continue;                                                                    \
+                    if (!hit[p]) {                                                                          \
+                        out[p] = add_id_##SUF(add);                                                         \
+                        hit[p] = 1;                                                                         \
+                    }                                                                                       \
+                    out[p] = add_##SUF(add, out[p], mul_##SUF(mul, aval[e], bval[q]));                      \
+                }                                                                                           \
+            }                                                                                               \
+            for (int64_t p = mrp[i]; p < mrp[i + 1]; ++p) {                                                 \
+                pos[mcol[p]] = -1;                                                                          \
+                if (!hit[p]) out[p] = (T)0;                                                                 \
+            }                                                                                               \
+        }                                                                                                   \
+        free(pos);                                                                                          \
+        free(hit);                                                                                          \
+    }
+ORC_MXM(f32, float)
+ORC_MXM(f64, double)
+ORC_MXM(i32, int32_t)
+ORC_MXM(i64, int64_t)
+ORC_MXM(u8, uint8_t)
+
+int orc_mxm_masked(int type, int64_t n, const int64_t* mrp, const int32_t* mcol, const int64_t* arp,
+                   const int32_t* acol, const void* aval, const int64_t* brp, const int32_t* bcol, const void* bval,
+                   int add, int mul, void* out) {
+    switch (type) {
+        case T_F32: mxm_f32(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
+        case T_F64: mxm_f64(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
+        case T_I32: mxm_i32(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
+        case T_I64: mxm_i64(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
+        case T_BOOL: mxm_u8(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
+    }
+    return -1;
+}
+
+/* Sum over the entries of C<M> = A (plus.one) B (the triangle-count kernel  */
+/* of LAGr_TriangleCount, algorithms/tc/tc.hpp:80-291).                     */
+int64_t orc_masked_count(int64_t n, const int64_t* mrp, const int32_t* mcol, const int64_t* arp,
+                         const int32_t* acol, const int64_t* brp, const int32_t* bcol) {
+    char* mark = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t p = mrp[i]; p < mrp[i + 1]; ++p) mark[mcol[p]] = 1;
+        for (int64_t e = arp[i]; e < arp[i + 1]; ++e) {
+            const int64_t k = acol[e];
+            for (int64_t q = brp[k]; q < brp[k + 1]; ++q) total += mark[bcol[q]];
+        }
+        for (int64_t p = mrp[i]; p < mrp[i + 1]; ++p) mark[mcol[p]] = 0;
+    }
+    free(mark);
+    return total;
+}

@@ -255,6 +255,31 @@ def ref_bfs(rp, col, source):
     return lv
 
 
+def mxm_masked(mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul):
+    """C<M> = A (+.*) B (the masked IKJ product): values in the mask's entry order."""
+    mrp, mcol = _csr(mrp, mcol)
+    arp, acol = _csr(arp, acol)
+    brp, bcol = _csr(brp, bcol)
+    aval = np.ascontiguousarray(aval)
+    bval = np.ascontiguousarray(bval, dtype=aval.dtype)
+    n = len(mrp) - 1
+    out = np.zeros(max(len(mcol), 1), dtype=aval.dtype)
+    rc = lib().orc_mxm_masked(TYPE_OF[aval.dtype], C.c_int64(n), _p(mrp), _p(mcol), _p(arp), _p(acol), _p(aval),
+                              _p(brp), _p(bcol), _p(bval), add, mul, _p(out))
+    assert rc == 0
+    return out[:len(mcol)]
+
+
+def masked_count(mrp, mcol, arp, acol, brp, bcol):
+    """Sum over the entries of C<M> = A (plus.one) B."""
+    mrp, mcol = _csr(mrp, mcol)
+    arp, acol = _csr(arp, acol)
+    brp, bcol = _csr(brp, bcol)
+    f = lib().orc_masked_count
+    f.restype = C.c_int64
+    return int(f(C.c_int64(len(mrp) - 1), _p(mrp), _p(mcol), _p(arp), _p(acol), _p(brp), _p(bcol)))
+
+
 def ref_read_mtx(path):
     """The compiled reference's Matrix::init_from_mtx: (row_ptr, col, seconds) of its CSR."""
     nr, nnz, sec = C.c_int64(0), C.c_int64(0), C.c_double(0)

@@ -311,3 +311,60 @@ def test_dijkstra_f32_matches_integer_dijkstra(oracle, gbk):
         reach = di != np.iinfo(np.int32).max
         assert np.array_equal(df[reach].astype(np.int64), di[reach].astype(np.int64))
         assert np.all(df[~reach] == np.finfo(np.float32).max)
+
+
+# ------------------------------------------------------------------ mxm / tc
+def _dense_csr(n, density, seed, self_loops=True):
+    rng = np.random.default_rng(seed)
+    d = rng.random((n, n)) < density
+    if not self_loops:
+        np.fill_diagonal(d, False)
+    rp = np.concatenate([[0], np.cumsum(d.sum(1))]).astype(np.int64)
+    col = np.nonzero(d)[1].astype(np.int32)
+    return d, rp, col
+
+
+def test_oracle_mxm_masked_matches_dense_products():
+    from oracle import oracle as O
+    n = 60
+    da, arp, acol = _dense_csr(n, 0.15, 1)
+    db, brp, bcol = _dense_csr(n, 0.15, 2)
+    dm, mrp, mcol = _dense_csr(n, 0.3, 3)
+    rng = np.random.default_rng(4)
+    av = rng.integers(1, 50, len(acol)).astype(np.int32)
+    bv = rng.integers(1, 50, len(bcol)).astype(np.int32)
+    A = np.zeros((n, n), np.int64); A[da] = av
+    B = np.zeros((n, n), np.int64); B[db] = bv
+    hits = (da.astype(np.int64) @ db.astype(np.int64)) > 0
+    mi, mj = np.nonzero(dm)
+    # plus.times
+    got = O.mxm_masked(mrp, mcol, arp, acol, av, brp, bcol, bv, 0, 0)
+    want = np.where(hits, A @ B, 0)[mi, mj]
+    assert np.array_equal(got.astype(np.int64), want)
+    # min.plus: min over the contributing k of A(i,k) + B(k,j)
+    big = np.iinfo(np.int64).max // 4
+    Ai = np.where(da, A, big)
+    Bi = np.where(db, B, big)
+    mp = np.min(Ai[:, :, None] + Bi[None, :, :], axis=1)
+    got = O.mxm_masked(mrp, mcol, arp, acol, av, brp, bcol, bv, 2, 4)
+    want = np.where(hits, mp, 0)[mi, mj]
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+def test_oracle_masked_count_is_the_triangle_count():
+    import scipy.sparse as sp
+    from oracle import oracle as O
+    for seed in (5, 6, 7):
+        d, _, _ = _dense_csr(120, 0.08, seed, self_loops=False)
+        d = d | d.T
+        S = sp.csr_matrix(d.astype(np.int64))
+        tri = int((S @ S @ S).diagonal().sum()) // 6
+        rp, col = S.indptr.astype(np.int64), S.indices.astype(np.int32)
+        L = sp.tril(S, -1).tocsr()
+        U = sp.triu(S, 1).tocsr()
+        lrp, lcol = L.indptr.astype(np.int64), L.indices.astype(np.int32)
+        urp, ucol = U.indptr.astype(np.int64), U.indices.astype(np.int32)
+        assert O.masked_count(rp, col, rp, col, rp, col) == 6 * tri              # Burkhardt
+        assert O.masked_count(rp, col, lrp, lcol, urp, ucol) == 2 * tri          # Cohen
+        assert O.masked_count(lrp, lcol, lrp, lcol, lrp, lcol) == tri            # Sandia
+        assert O.masked_count(urp, ucol, urp, ucol, urp, ucol) == tri            # Sandia2
