@@ -56,12 +56,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--bfs-scale", type=int, default=23, help="scale of the symmetric R-MAT for BFS / SSSP GTEPS")
+    ap.add_argument("--tc-scale", type=int, default=20, help="scale of the symmetric R-MAT for triangle counting")
     ap.add_argument("--serve-full", action="store_true",
                     help="served measurement on the full 8-graph set (R-MAT 18..24 + ER 22, 20,000 queries)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed loop, copy this many MiB device-to-device on the library stream "
                          "(a kernel of known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal", "io"],
+    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal", "io", "tc"],
                     help="run one secondary measurement alone (for profiling); prints that section as JSON")
     args = ap.parse_args()
 
@@ -81,6 +82,8 @@ def main():
             res = served_bench(g, full=args.serve_full)
         elif args.section == "io":
             res = io_bench(g, args.scale, args.edge_factor)
+        elif args.section == "tc":
+            res = tc_bench(g, args.tc_scale)
         else:
             res = traversal_bench(g, args.bfs_scale)
         print(json.dumps({"section": args.section, args.section: res}), flush=True)
@@ -430,6 +433,37 @@ def served_bench(g, full=False, seed=11):
     for G in graphs:
         for m in G["m"]:
             m.free()
+    return res
+
+
+def tc_bench(g, scale):
+    """Triangle counting (LAGr_TriangleCount) on a symmetric R-MAT: time of the
+    default (Sandia over the degree orientation), Burkhardt (A^2 .* A) and
+    Sandia (L L .* L) masked products, and the oracle's sequential masked
+    count of Sandia on the host."""
+    from oracle import oracle as O
+    import scipy.sparse as sp
+    rp, col = g.gen_rmat(scale, 16, seed=47, symmetric=True)
+    n = len(rp) - 1
+    A = g.Matrix.from_csr(rp, col, np.ones(len(col), np.int32))
+    res = {"graph": f"symmetric R-MAT scale {scale}, edge factor 16", "n": n, "stored_nnz": len(col)}
+    for name, m in (("default_degree_oriented", 0), ("burkhardt", 1), ("sandia", 3)):
+        g.tricount(A, m)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            tri = g.tricount(A, m)
+            times.append(time.perf_counter() - t0)
+        res[name] = {"triangles": tri, "ms_median": round(float(np.median(times)) * 1e3, 2)}
+        log(f"tricount {name}: {tri} triangles in {np.median(times) * 1e3:.1f} ms")
+    S = sp.csr_matrix((np.ones(len(col), np.int8), col, rp), shape=(n, n))
+    L = sp.tril(S, -1).tocsr()
+    t0 = time.perf_counter()
+    cpu = O.masked_count(L.indptr, L.indices, L.indptr, L.indices, L.indptr, L.indices)
+    tc = time.perf_counter() - t0
+    res["cpu_sandia"] = {"triangles": cpu, "seconds": round(tc, 3), "cores": 1, "kind": "port"}
+    log(f"oracle (1 core) Sandia: {cpu} triangles in {tc:.2f} s")
+    A.free()
     return res
 
 

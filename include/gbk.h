@@ -362,7 +362,8 @@ int gbk_mxm(gbk_matrix C, gbk_matrix M, int accum, gbk_semiring op, gbk_matrix A
             const gbk_desc* desc);
 
 /* Triangle count of a symmetric matrix, replaces LAGr_TriangleCount
- * (algorithms/tc/tc.hpp:80-291).  method: 0 default (= 1), 1 Burkhardt
+ * (algorithms/tc/tc.hpp:80-291).  method: 0 default (Sandia over the
+ * orientation by (degree, id); the reference maps 0 to 1), 1 Burkhardt
  * sum((A^2) .* A) / 6, 2 Cohen sum((L U) .* A) / 2, 3 Sandia sum((L L) .* L),
  * 4 Sandia2 sum((U U) .* U), 5 SandiaDot sum((L U') .* L), 6 SandiaDot2
  * sum((U L') .* U), L = tril(A, -1), U = triu(A, 1).  All give the same count
