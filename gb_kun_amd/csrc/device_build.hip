@@ -85,6 +85,38 @@ int dev_sort_pairs(const uint32_t* kin, uint32_t* kout, const int32_t* vin, int3
     return GBK_SUCCESS;
 }
 
+// The same with 64-bit keys (bits up to 64).
+int dev_sort_pairs64(const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout, int64_t m, int bits,
+                     hipStream_t st) {
+    if (m <= 0) return GBK_SUCCESS;
+    size_t tmp_bytes = 0;
+    GBK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, vout, (size_t)m, 0, bits, st));
+    void* tmp = nullptr;
+    GBK_TRY(dev_alloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1));
+    const hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, vout, (size_t)m, 0, bits, st);
+    const hipError_t e2 = hipStreamSynchronize(st);
+    dev_free(tmp);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(GBK_PANIC, "radix sort failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    return GBK_SUCCESS;
+}
+
+// out[i] = sum_{j<i} in[j] over m int64 elements (rocPRIM scan).
+int dev_exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t m, hipStream_t st) {
+    if (m <= 0) return GBK_SUCCESS;
+    size_t tmp_bytes = 0;
+    GBK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, in, out, (int64_t)0, (size_t)m, rocprim::plus<int64_t>(), st));
+    void* tmp = nullptr;
+    GBK_TRY(dev_alloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1));
+    const hipError_t e =
+        rocprim::exclusive_scan(tmp, tmp_bytes, in, out, (int64_t)0, (size_t)m, rocprim::plus<int64_t>(), st);
+    const hipError_t e2 = hipStreamSynchronize(st);
+    dev_free(tmp);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(GBK_PANIC, "scan failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    return GBK_SUCCESS;
+}
+
 int dev_iota(int32_t* v, int64_t n, hipStream_t st) {
     if (n <= 0) return GBK_SUCCESS;
     hipLaunchKernelGGL(k_iota, grid_n(n), dim3(256), 0, st, v, n);

@@ -1,5 +1,5 @@
 // Optimized matrix representations (SELL-C-sigma, segmented CSR, LAV) on the
-// device, built from the CSR / CSC by format_build.cpp, one object per
+// device, built on the device from the CSR / CSC, one object per
 // orientation as the reference keeps data / transposed_data
 // (src/backend/matrix/build.hpp:6-118).
 //
@@ -21,7 +21,6 @@
 #include <vector>
 
 #include "gbk_objects.h"
-#include "host_formats.h"
 
 namespace gbk {
 
@@ -88,21 +87,6 @@ template <typename V> static int upload(V** dst, const std::vector<V>& src) {
     return GBK_SUCCESS;
 }
 
-static int download_csr(const DevCSR& m, int type, std::vector<int32_t>& rp, std::vector<int32_t>& col,
-                        std::vector<char>& val) {
-    const size_t vs = type_size(type);
-    rp.resize(m.nrows + 1);
-    col.resize(m.nnz);
-    val.resize(vs * m.nnz);
-    GBK_HIP(hipStreamSynchronize(stream()));
-    GBK_HIP(hipMemcpy(rp.data(), m.row_ptr, sizeof(int32_t) * (m.nrows + 1), hipMemcpyDeviceToHost));
-    if (m.nnz) {
-        GBK_HIP(hipMemcpy(col.data(), m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
-        GBK_HIP(hipMemcpy(val.data(), m.val, vs * m.nnz, hipMemcpyDeviceToHost));
-    }
-    return GBK_SUCCESS;
-}
-
 // ------------------------------------------------------------ SELL-C-sigma --
 static void free_sell(SellC* s) {
     if (!s) return;
@@ -122,54 +106,295 @@ static void free_sell(SellC* s) {
     delete s;
 }
 
-static int build_sell(const DevCSR& m, int type, int C, int sigma, SellC** out) {
-    std::vector<int32_t> rp, col;
-    std::vector<char> val;
-    GBK_TRY(download_csr(m, type, rp, col, val));
-    gbk_host::SellCHost h;
-    // rows longer than 4 C entries go to the merge-path kernel
-    gbk_host::build_sell_c(m.nrows, rp.data(), col.data(), val.data(), type_size(type), C, sigma, 4 * (int64_t)C, h);
-    std::sort(h.long_rows.begin(), h.long_rows.end());
-    SellC* s = new SellC();
-    s->C = C;
-    s->sigma = sigma;
-    s->nchunks = h.nchunks;
-    s->padded = h.padded;
-    s->nproblem = (int64_t)h.problem_rows.size();
-    s->nlong = (int64_t)h.long_rows.size();
-    int rc = upload(&s->chunk_ptr, h.chunk_ptr);
-    if (rc == GBK_SUCCESS) rc = upload(&s->chunk_len, h.chunk_len);
-    if (rc == GBK_SUCCESS) rc = upload(&s->row_perm, h.row_perm);
-    if (rc == GBK_SUCCESS) rc = upload(&s->row_len, h.row_len);
-    if (rc == GBK_SUCCESS) rc = upload(&s->col, h.col);
-    if (rc == GBK_SUCCESS) rc = upload((char**)&s->val, h.val);
-    if (rc == GBK_SUCCESS) rc = upload(&s->problem_rows, h.problem_rows);
-    if (rc == GBK_SUCCESS) rc = upload(&s->long_rows, h.long_rows);
-    if (rc == GBK_SUCCESS && !h.long_rows.empty()) {
-        // CSR of the long rows, in ascending row order
-        const size_t vs = type_size(type);
-        const int64_t nl = (int64_t)h.long_rows.size();
-        std::vector<int32_t> lrp(nl + 1, 0), lcol;
-        std::vector<char> lval;
-        for (int64_t i = 0; i < nl; ++i) {
-            const int32_t r = h.long_rows[i];
-            lrp[i + 1] = lrp[i] + (rp[r + 1] - rp[r]);
-        }
-        lcol.resize(lrp[nl]);
-        lval.resize((size_t)lrp[nl] * vs);
-        for (int64_t i = 0; i < nl; ++i) {
-            const int32_t r = h.long_rows[i];
-            std::copy(col.begin() + rp[r], col.begin() + rp[r + 1], lcol.begin() + lrp[i]);
-            std::copy(val.begin() + (size_t)rp[r] * vs, val.begin() + (size_t)rp[r + 1] * vs, lval.begin() + (size_t)lrp[i] * vs);
-        }
-        s->lng.nrows = nl;
-        s->lng.ncols = m.ncols;
-        s->lng.nnz = lrp[nl];
-        rc = upload(&s->lng.row_ptr, lrp);
-        if (rc == GBK_SUCCESS) rc = upload(&s->lng.col, lcol);
-        if (rc == GBK_SUCCESS) rc = upload((char**)&s->lng.val, lval);
-        if (rc == GBK_SUCCESS) rc = build_merge_tiles(s->lng, type);
+// Device -> host copy ordered after the work queued on st (the library
+// stream does not synchronize with the null stream).
+static int d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    GBK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}
+
+// Device scratch of one build step, released when it leaves scope.
+struct TmpBuf {
+    void* p = nullptr;
+    ~TmpBuf() { dev_free(p); }
+    int alloc(size_t bytes) { return dev_alloc(&p, bytes > 0 ? bytes : 1); }
+    template <typename V> V* as() const { return (V*)p; }
+};
+
+// ------------------------------------------------- SELL-C-sigma, device build --
+// Replaces MatrixSellC::build / construct_sell_c_sigma
+// (src/backend/matrix/containers/sell_c/build.hpp:28-295), which runs on one
+// host thread.  Rows are sorted by length, descending and stable, inside
+// windows of sigma rows (a trailing window only when it holds more than C
+// rows, build.hpp:92-97) by one radix sort of (window, ~length) keys; chunks
+// of C sorted rows are stored column-major, element j of lane r at
+// chunk_ptr[c] + j*C + r, padded with column -1 to the chunk's longest row.
+// A chunk whose padded size is at least twice its entries and which holds
+// more than 1024 entries is "problematic" (build.hpp:147-155): it is not
+// stored, and its rows are listed for a row kernel (the reference's fallback
+// loop returns the identity for them, a defect not reproduced).  Rows longer
+// than long_thr leave their chunks the same way (listed ascending, with a CSR
+// of their own for the merge-path kernel), so no lane walks a hub row alone.
+
+__global__ void k_sell_key(const int32_t* __restrict__ rp, int64_t n, int64_t sigma, int64_t nwin, int sort_rest,
+                           uint64_t* __restrict__ key) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t len = (uint32_t)(rp[r + 1] - rp[r]);
+    const int64_t w = sigma > 0 ? min(r / sigma, nwin) : 0;
+    const bool sorted = sigma > 0 && (w < nwin || sort_rest);
+    key[r] = ((uint64_t)w << 32) | (sorted ? (uint64_t)(0xFFFFFFFFu - len) : 0);
+}
+
+// Sorted position p -> row and its length (-1 / 0 past n).
+__global__ void k_sell_place(const int32_t* __restrict__ perm, const int32_t* __restrict__ rp, int64_t n, int64_t np,
+                             int32_t* __restrict__ row_perm, int32_t* __restrict__ row_len) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    if (p < n) {
+        const int32_t row = perm[p];
+        row_perm[p] = row;
+        row_len[p] = rp[row + 1] - rp[row];
+    } else {
+        row_perm[p] = -1;
+        row_len[p] = 0;
     }
+}
+
+// One thread per chunk: its padded length over the rows it keeps, and
+// whether it is problematic.  csize[c] = stored slots (chunk_len * C).
+__global__ void k_sell_chunks(const int32_t* __restrict__ row_len, int64_t n, int64_t nchunks, int C,
+                              int64_t long_thr, int32_t* __restrict__ chunk_len, int64_t* __restrict__ csize,
+                              uint8_t* __restrict__ prob) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    int64_t mx = 0, tot = 0;
+    for (int r = 0; r < C; ++r) {
+        const int64_t p = c * C + r;
+        if (p >= n) break;
+        const int64_t L = row_len[p];
+        if (L > long_thr) continue;
+        mx = max(mx, L);
+        tot += L;
+    }
+    const bool pr = mx * C >= 2 * tot && tot > 1024;
+    prob[c] = pr ? 1 : 0;
+    chunk_len[c] = pr ? 0 : (int32_t)mx;
+    csize[c] = pr ? 0 : mx * C;
+}
+
+// pflag[p] = 1 for a short row of a problematic chunk; lflag[row] = 1 for a long row.
+__global__ void k_sell_flags(const int32_t* __restrict__ row_perm, const int32_t* __restrict__ row_len,
+                             const uint8_t* __restrict__ prob, int64_t n, int C, int64_t long_thr,
+                             int32_t* __restrict__ pflag, int32_t* __restrict__ lflag) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const bool lng = row_len[p] > long_thr;
+    pflag[p] = (!lng && prob[p / C]) ? 1 : 0;
+    if (lng) lflag[row_perm[p]] = 1;
+}
+
+// Lists the problematic rows (position order) and takes listed rows out of
+// the chunks.
+__global__ void k_sell_unlink(int32_t* __restrict__ row_perm, int32_t* __restrict__ row_len,
+                              const int32_t* __restrict__ pflag, const int32_t* __restrict__ pidx, int64_t n,
+                              int64_t long_thr, int32_t* __restrict__ problem_rows) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (pflag[p]) problem_rows[pidx[p]] = row_perm[p];
+    if (pflag[p] || row_len[p] > long_thr) {
+        row_perm[p] = -1;
+        row_len[p] = 0;
+    }
+}
+
+__global__ void k_list_flagged(const int32_t* __restrict__ flag, const int32_t* __restrict__ idx, int64_t n,
+                               int32_t* __restrict__ list) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) list[idx[i]] = (int32_t)i;
+}
+
+// One thread per slot position: writes the chunk column of its row, padding
+// (column -1, value 0) past the row's end.
+template <typename V>
+__global__ void k_sell_fill(const int64_t* __restrict__ chunk_ptr, const int32_t* __restrict__ chunk_len,
+                            const int32_t* __restrict__ row_perm, const int32_t* __restrict__ row_len,
+                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const V* __restrict__ val, int64_t np, int C, int32_t* __restrict__ scol,
+                            V* __restrict__ sval) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int64_t c = p / C;
+    const int r = (int)(p % C);
+    const int32_t L = chunk_len[c];
+    const int32_t row = row_perm[p];
+    const int32_t rl = row >= 0 ? row_len[p] : 0;
+    const int64_t start = row >= 0 ? rp[row] : 0;
+    const int64_t base = chunk_ptr[c] + r;
+    for (int32_t j = 0; j < L; ++j) {
+        const int64_t e = base + (int64_t)j * C;
+        if (j < rl) {
+            scol[e] = col[start + j];
+            sval[e] = val[start + j];
+        } else {
+            scol[e] = -1;
+            sval[e] = V(0);
+        }
+    }
+}
+
+__global__ void k_list_len(const int32_t* __restrict__ rows, const int32_t* __restrict__ rp, int64_t m,
+                           int32_t* __restrict__ len) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) len[i] = rp[rows[i] + 1] - rp[rows[i]];
+}
+
+// One wave per listed row: copies its entries into the sub-CSR.
+template <typename V>
+__global__ void k_rows_copy(const int32_t* __restrict__ rows, const int32_t* __restrict__ rp,
+                            const int32_t* __restrict__ col, const V* __restrict__ val, int64_t m,
+                            const int32_t* __restrict__ lrp, int32_t* __restrict__ lcol, V* __restrict__ lval) {
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (i >= m) return;
+    const int32_t r = rows[i];
+    const int64_t s = rp[r], L = rp[r + 1] - s, d = lrp[i];
+    for (int64_t j = lane; j < L; j += 64) {
+        lcol[d + j] = col[s + j];
+        lval[d + j] = val[s + j];
+    }
+}
+
+static dim3 grid_b(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return dim3((unsigned)(b > 0 ? b : 1));
+}
+
+template <typename F> static int by_value_size(size_t vs, F&& f) {
+    if (vs == 8) return f(uint64_t());
+    if (vs == 4) return f(uint32_t());
+    if (vs == 1) return f(uint8_t());
+    return fail(GBK_INVALID_VALUE, "unsupported value size %zu", vs);
+}
+
+static int build_sell_device(const DevCSR& m, int type, int C, int sigma, int64_t long_thr, SellC& s) {
+    hipStream_t st = stream();
+    const size_t vs = type_size(type);
+    const int64_t n = m.nrows;
+    const int64_t nchunks = (n + C - 1) / C;
+    const int64_t np = nchunks * C;
+    s.C = C;
+    s.sigma = sigma;
+    s.nchunks = nchunks;
+    const int64_t n1 = n > 0 ? n : 1, np1 = np > 0 ? np : 1;
+    GBK_TRY(dev_alloc((void**)&s.chunk_ptr, 8 * (nchunks + 1)));
+    GBK_TRY(dev_alloc((void**)&s.chunk_len, 4 * (nchunks > 0 ? nchunks : 1)));
+    GBK_TRY(dev_alloc((void**)&s.row_perm, 4 * np1));
+    GBK_TRY(dev_alloc((void**)&s.row_len, 4 * np1));
+    if (n == 0) {
+        GBK_HIP(hipMemsetAsync(s.chunk_ptr, 0, 8, st));
+        GBK_TRY(dev_alloc((void**)&s.col, 4));
+        GBK_TRY(dev_alloc(&s.val, vs));
+        GBK_HIP(hipStreamSynchronize(st));
+        return GBK_SUCCESS;
+    }
+    // sigma-window sort of the rows
+    {
+        const int64_t nwin = sigma > 0 ? n / sigma : 0;
+        const int64_t rest = sigma > 0 ? n % sigma : 0;
+        TmpBuf key, key_out, ids, perm;
+        GBK_TRY(key.alloc(8 * n1));
+        GBK_TRY(key_out.alloc(8 * n1));
+        GBK_TRY(ids.alloc(4 * n1));
+        GBK_TRY(perm.alloc(4 * n1));
+        hipLaunchKernelGGL(k_sell_key, grid_b(n), dim3(256), 0, st, m.row_ptr, n, (int64_t)sigma, nwin,
+                           rest > C ? 1 : 0, key.as<uint64_t>());
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(dev_iota(ids.as<int32_t>(), n, st));
+        GBK_TRY(dev_sort_pairs64(key.as<uint64_t>(), key_out.as<uint64_t>(), ids.as<int32_t>(), perm.as<int32_t>(),
+                                 n, 32 + bits_for(nwin), st));
+        hipLaunchKernelGGL(k_sell_place, grid_b(np), dim3(256), 0, st, perm.as<int32_t>(), m.row_ptr, n, np,
+                           s.row_perm, s.row_len);
+        GBK_LAUNCH_CHECK();
+    }
+    // chunk lengths, problematic chunks, chunk pointers
+    TmpBuf csize, prob, pflag, pidx, lflag, lidx;
+    GBK_TRY(csize.alloc(8 * (nchunks + 1)));
+    GBK_TRY(prob.alloc(nchunks));
+    hipLaunchKernelGGL(k_sell_chunks, grid_b(nchunks), dim3(256), 0, st, s.row_len, n, nchunks, C, long_thr,
+                       s.chunk_len, csize.as<int64_t>(), prob.as<uint8_t>());
+    GBK_LAUNCH_CHECK();
+    GBK_HIP(hipMemsetAsync(csize.as<int64_t>() + nchunks, 0, 8, st));
+    GBK_TRY(dev_exclusive_scan_i64(csize.as<int64_t>(), s.chunk_ptr, nchunks + 1, st));
+    GBK_TRY(d2h(&s.padded, s.chunk_ptr + nchunks, 8, st));
+    // listed rows: short rows of problematic chunks, long rows
+    GBK_TRY(pflag.alloc(4 * n1));
+    GBK_TRY(pidx.alloc(4 * (n1 + 1)));
+    GBK_TRY(lflag.alloc(4 * (n1 + 1)));
+    GBK_TRY(lidx.alloc(4 * (n1 + 1)));
+    GBK_HIP(hipMemsetAsync(lflag.p, 0, 4 * (n1 + 1), st));
+    hipLaunchKernelGGL(k_sell_flags, grid_b(n), dim3(256), 0, st, s.row_perm, s.row_len, prob.as<uint8_t>(), n, C,
+                       long_thr, pflag.as<int32_t>(), lflag.as<int32_t>());
+    GBK_LAUNCH_CHECK();
+    int32_t npr = 0, nlg = 0;
+    GBK_TRY(exclusive_scan_i32(pflag.as<int32_t>(), pidx.as<int32_t>(), n, &npr));
+    GBK_TRY(exclusive_scan_i32(lflag.as<int32_t>(), lidx.as<int32_t>(), n, &nlg));
+    s.nproblem = npr;
+    s.nlong = nlg;
+    GBK_TRY(dev_alloc((void**)&s.problem_rows, 4 * (npr > 0 ? npr : 1)));
+    GBK_TRY(dev_alloc((void**)&s.long_rows, 4 * (nlg > 0 ? nlg : 1)));
+    hipLaunchKernelGGL(k_sell_unlink, grid_b(n), dim3(256), 0, st, s.row_perm, s.row_len, pflag.as<int32_t>(),
+                       pidx.as<int32_t>(), n, long_thr, s.problem_rows);
+    GBK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_list_flagged, grid_b(n), dim3(256), 0, st, lflag.as<int32_t>(), lidx.as<int32_t>(), n,
+                       s.long_rows);
+    GBK_LAUNCH_CHECK();
+    // the chunk slots
+    const int64_t pad1 = s.padded > 0 ? s.padded : 1;
+    GBK_TRY(dev_alloc((void**)&s.col, 4 * pad1));
+    GBK_TRY(dev_alloc(&s.val, vs * pad1));
+    GBK_TRY(by_value_size(vs, [&](auto tv) -> int {
+        typedef decltype(tv) V;
+        hipLaunchKernelGGL(k_sell_fill<V>, grid_b(np), dim3(256), 0, st, s.chunk_ptr, s.chunk_len, s.row_perm,
+                           s.row_len, m.row_ptr, m.col, (const V*)m.val, np, C, s.col, (V*)s.val);
+        GBK_LAUNCH_CHECK();
+        return GBK_SUCCESS;
+    }));
+    // CSR of the long rows, ascending
+    if (nlg > 0) {
+        TmpBuf llen;
+        GBK_TRY(llen.alloc(4 * nlg));
+        DevCSR& L = s.lng;
+        GBK_TRY(dev_alloc((void**)&L.row_ptr, 4 * (nlg + 1)));
+        hipLaunchKernelGGL(k_list_len, grid_b(nlg), dim3(256), 0, st, s.long_rows, m.row_ptr, (int64_t)nlg,
+                           llen.as<int32_t>());
+        GBK_LAUNCH_CHECK();
+        int32_t lnnz = 0;
+        GBK_TRY(exclusive_scan_i32(llen.as<int32_t>(), L.row_ptr, nlg, &lnnz));
+        L.nrows = nlg;
+        L.ncols = m.ncols;
+        L.nnz = lnnz;
+        GBK_TRY(dev_alloc((void**)&L.col, 4 * (lnnz > 0 ? lnnz : 1)));
+        GBK_TRY(dev_alloc(&L.val, vs * (lnnz > 0 ? lnnz : 1)));
+        GBK_TRY(by_value_size(vs, [&](auto tv) -> int {
+            typedef decltype(tv) V;
+            hipLaunchKernelGGL(k_rows_copy<V>, grid_b((int64_t)nlg * 64), dim3(256), 0, st, s.long_rows, m.row_ptr,
+                               m.col, (const V*)m.val, (int64_t)nlg, L.row_ptr, L.col, (V*)L.val);
+            GBK_LAUNCH_CHECK();
+            return GBK_SUCCESS;
+        }));
+        GBK_TRY(build_merge_tiles(L, type));
+    }
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}
+
+static int build_sell(const DevCSR& m, int type, int C, int sigma, SellC** out) {
+    if (C <= 0 || C > 64) return fail(GBK_INVALID_VALUE, "SELL-C chunk size %d out of range (1..64)", C);
+    SellC* s = new SellC();
+    // rows longer than 4 C entries go to the merge-path kernel
+    const int rc = build_sell_device(m, type, C, sigma, 4 * (int64_t)C, *s);
     if (rc != GBK_SUCCESS) {
         free_sell(s);
         return rc;
@@ -301,27 +526,6 @@ static std::vector<int32_t> xcd_tile_map(const std::vector<int2>& tiles, const s
     return map;
 }
 
-static int build_seg_common(const gbk_host::SegHost& h, int type, int64_t n, SegCSR& s) {
-    const size_t vs = type_size(type);
-    s.nseg = h.nseg;
-    s.npairs = h.npairs;
-    s.n = n;
-    s.v.nrows = s.v.ncols = h.npairs;
-    s.v.nnz = (int64_t)h.col.size();
-    GBK_TRY(upload(&s.v.row_ptr, h.vrp));
-    GBK_TRY(upload(&s.v.col, h.col));
-    GBK_TRY(upload((char**)&s.v.val, h.val));
-    GBK_TRY(build_merge_tiles(s.v, type));
-    std::vector<int2> tiles(s.v.ntiles + 1);
-    GBK_HIP(hipMemcpy(tiles.data(), s.v.tiles, sizeof(int2) * (s.v.ntiles + 1), hipMemcpyDeviceToHost));
-    GBK_TRY(upload(&s.tile_map, xcd_tile_map(tiles, h.seg_pair_start, h.nseg)));
-    GBK_TRY(upload(&s.rpp, h.rpp));
-    GBK_TRY(upload(&s.row_pairs, h.row_pairs));
-    GBK_TRY(dev_alloc(&s.partial, vs * (h.npairs > 0 ? h.npairs : 1)));
-    (void)vs;
-    return GBK_SUCCESS;
-}
-
 static int64_t seg_cols_default(int64_t n) {
     // 2 MiB of fp32 x per segment: half of one XCD's 4 MiB L2
     (void)n;
@@ -354,13 +558,6 @@ static int hot_slots(size_t vs) {
 // columns are sorted by (segment, descending in-degree, id) for the slots.
 // Replaces MatrixSegmentedCSR::build (seg_csr/build.hpp:24-169), which runs
 // single-threaded on the host.
-
-struct TmpBuf {
-    void* p = nullptr;
-    ~TmpBuf() { dev_free(p); }
-    int alloc(size_t bytes) { return dev_alloc(&p, bytes > 0 ? bytes : 1); }
-    template <typename V> V* as() const { return (V*)p; }
-};
 
 __global__ void k_seg_key(const int32_t* __restrict__ col, int64_t nnz, int64_t w, uint32_t* __restrict__ key) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,7 +661,21 @@ static dim3 grid_e(int64_t n) {
     return dim3((unsigned)(b > 0 ? b : 1));
 }
 
-static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, SegCSR& s) {
+// v.col[i] = relabel[col[perm[i]]] (col[perm[i]] without a relabelling).
+__global__ void k_gather_cols(const int32_t* __restrict__ col, const int32_t* __restrict__ perm,
+                              const int32_t* __restrict__ relabel, int64_t nnz, int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const int32_t c = col[perm[i]];
+    out[i] = relabel ? relabel[c] : c;
+}
+
+// The (segment, row) pairs of a segmented CSR whose entry e lies in segment
+// ekey[e]: the virtual CSR (columns relabelled when `relabel` is given), the
+// row -> pairs map and the pair sums' buffer.  seg_pair receives the first
+// pair of every segment.
+static int seg_pairs_device(const DevCSR& m, int type, int64_t nseg, const uint32_t* ekey, const int32_t* relabel,
+                            SegCSR& s, std::vector<int32_t>& seg_pair) {
     hipStream_t st = stream();
     const size_t vs = type_size(type);
     const int64_t n = m.nrows, nnz = m.nnz;
@@ -472,20 +683,15 @@ static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, 
     s.nseg = (int)nseg;
     s.n = n;
     // entries stably sorted by segment
-    TmpBuf key, skey, iota, perm, rowid, flag, pidx;
-    GBK_TRY(key.alloc(4 * z1));
+    TmpBuf skey, iota, perm, rowid, flag, pidx;
     GBK_TRY(skey.alloc(4 * z1));
     GBK_TRY(iota.alloc(4 * z1));
     GBK_TRY(perm.alloc(4 * z1));
     GBK_TRY(rowid.alloc(4 * z1));
     GBK_TRY(flag.alloc(4 * z1));
     GBK_TRY(pidx.alloc(4 * (z1 + 1)));  // the scan also writes the total at [nnz]
-    if (nnz > 0) {
-        hipLaunchKernelGGL(k_seg_key, grid_e(nnz), dim3(256), 0, st, m.col, nnz, w, key.as<uint32_t>());
-        GBK_LAUNCH_CHECK();
-    }
     GBK_TRY(dev_iota(iota.as<int32_t>(), nnz, st));
-    GBK_TRY(dev_sort_pairs(key.as<uint32_t>(), skey.as<uint32_t>(), iota.as<int32_t>(), perm.as<int32_t>(), nnz,
+    GBK_TRY(dev_sort_pairs(ekey, skey.as<uint32_t>(), iota.as<int32_t>(), perm.as<int32_t>(), nnz,
                            bits_for(nseg - 1), st));
     GBK_TRY(dev_row_ids(m.row_ptr, n, rowid.as<int32_t>(), st));
     if (nnz > 0) {
@@ -517,17 +723,21 @@ static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, 
     }
     hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(64), 0, st, s.v.row_ptr, npairs, (int32_t)nnz);
     GBK_LAUNCH_CHECK();
-    GBK_TRY(dev_gather_i32(m.col, perm.as<int32_t>(), nnz, s.v.col, st));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_gather_cols, grid_e(nnz), dim3(256), 0, st, m.col, perm.as<int32_t>(), relabel, nnz,
+                           s.v.col);
+        GBK_LAUNCH_CHECK();
+    }
     GBK_TRY(dev_gather_bytes(m.val, perm.as<int32_t>(), nnz, vs, s.v.val, st));
     // first pair of every segment
-    TmpBuf segcnt, segpair_d;
+    TmpBuf segcnt;
     GBK_TRY(segcnt.alloc(4 * (nseg + 1)));
     GBK_HIP(hipMemsetAsync(segcnt.p, 0, 4 * (nseg + 1), st));
     GBK_TRY(dev_count(pair_seg.as<int32_t>(), npairs, segcnt.as<int32_t>(), st));
     std::vector<int32_t> sc(nseg + 1);
     GBK_HIP(hipMemcpyAsync(sc.data(), segcnt.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, st));
     GBK_HIP(hipStreamSynchronize(st));
-    std::vector<int32_t> seg_pair(nseg + 1, 0);
+    seg_pair.assign(nseg + 1, 0);
     for (int64_t g = 0; g < nseg; ++g) seg_pair[g + 1] = seg_pair[g] + sc[g];
     // rows -> their pairs in segment order: pairs stably sorted by row
     TmpBuf piota, pkey_out;
@@ -547,6 +757,26 @@ static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, 
         GBK_TRY(exclusive_scan_i32(rcnt.as<int32_t>(), s.rpp, n, &tot));  // rpp[0..n]
     }
     GBK_TRY(dev_alloc(&s.partial, vs * (npairs > 0 ? npairs : 1)));
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}
+
+static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, SegCSR& s) {
+    hipStream_t st = stream();
+    const size_t vs = type_size(type);
+    const int64_t n = m.nrows, nnz = m.nnz;
+    const int64_t z1 = nnz > 0 ? nnz : 1;
+    std::vector<int32_t> seg_pair;
+    {
+        TmpBuf key;
+        GBK_TRY(key.alloc(4 * z1));
+        if (nnz > 0) {
+            hipLaunchKernelGGL(k_seg_key, grid_e(nnz), dim3(256), 0, st, m.col, nnz, w, key.as<uint32_t>());
+            GBK_LAUNCH_CHECK();
+        }
+        GBK_TRY(seg_pairs_device(m, type, nseg, key.as<uint32_t>(), nullptr, s, seg_pair));
+    }
+    const int64_t npairs = s.npairs;
     // hot slots: columns by (segment, descending in-degree, id)
     SegHot& o = s.hot;
     const int H = hot_slots(vs);
@@ -923,20 +1153,103 @@ static int seg_spmv(const SegCSR* s, int type, const void* x, void* y, int add, 
 }
 
 // ------------------------------------------------------------------- LAV --
+// LAV, device build.  Replaces MatrixLAV::build (lav/build.hpp:103-212):
+// columns are relabelled by descending in-degree (stable by id, one radix
+// sort); the leading columns that cover 80% of the entries form dense
+// segments of 512 Ki columns, the rest one sparse segment; then it is the
+// segmented CSR over the relabelled columns, with x permuted on each call.
+
+__global__ void k_desc_key(const int32_t* __restrict__ deg, int64_t n, uint32_t* __restrict__ key) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) key[c] = 0xFFFFFFFFu - (uint32_t)deg[c];
+}
+
+__global__ void k_invert(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+// K = number of leading relabelled columns whose preceding entries number
+// fewer than `target`: the shortest prefix that covers it.
+__global__ void k_count_below(const int32_t* __restrict__ excl, int64_t n, double target, int32_t* __restrict__ K) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool below = i < n && (double)excl[i] < target;
+    const uint64_t b = __ballot(below);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(K, (int32_t)__popcll(b));
+}
+
+// Segment of entry e: relabelled column c' < K lies in dense segment
+// c' / w, the rest in the sparse segment ndense.
+__global__ void k_lav_key(const int32_t* __restrict__ col, const int32_t* __restrict__ o2n, int64_t nnz, int64_t K,
+                          int64_t w, int64_t ndense, uint32_t* __restrict__ key) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int64_t c = o2n[col[e]];
+    key[e] = (uint32_t)(c < K ? c / w : ndense);
+}
+
+static int build_lav_device(const DevCSR& m, int type, double dense_frac, int64_t w, Lav& l) {
+    hipStream_t st = stream();
+    const int64_t n = m.nrows, nnz = m.nnz;
+    const int64_t n1 = n > 0 ? n : 1, z1 = nnz > 0 ? nnz : 1;
+    GBK_TRY(dev_alloc((void**)&l.new_to_old, 4 * n1));
+    GBK_TRY(dev_alloc(&l.xperm, type_size(type) * n1));
+    TmpBuf deg, key, key_out, ids, o2n, sdeg, excl, Kd;
+    GBK_TRY(deg.alloc(4 * (n1 + 1)));
+    GBK_TRY(key.alloc(4 * n1));
+    GBK_TRY(key_out.alloc(4 * n1));
+    GBK_TRY(ids.alloc(4 * n1));
+    GBK_TRY(o2n.alloc(4 * n1));
+    GBK_TRY(sdeg.alloc(4 * n1));
+    GBK_TRY(excl.alloc(4 * (n1 + 1)));
+    GBK_TRY(Kd.alloc(4));
+    GBK_HIP(hipMemsetAsync(deg.p, 0, 4 * (n1 + 1), st));
+    GBK_HIP(hipMemsetAsync(Kd.p, 0, 4, st));
+    GBK_TRY(dev_count(m.col, nnz, deg.as<int32_t>(), st));
+    int32_t K = 0;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_desc_key, grid_b(n), dim3(256), 0, st, deg.as<int32_t>(), n, key.as<uint32_t>());
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(dev_iota(ids.as<int32_t>(), n, st));
+        GBK_TRY(dev_sort_pairs(key.as<uint32_t>(), key_out.as<uint32_t>(), ids.as<int32_t>(), l.new_to_old, n, 32,
+                               st));
+        hipLaunchKernelGGL(k_invert, grid_b(n), dim3(256), 0, st, l.new_to_old, n, o2n.as<int32_t>());
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(dev_gather_i32(deg.as<int32_t>(), l.new_to_old, n, sdeg.as<int32_t>(), st));
+        int32_t tot = 0;
+        GBK_TRY(exclusive_scan_i32(sdeg.as<int32_t>(), excl.as<int32_t>(), n, &tot));
+        hipLaunchKernelGGL(k_count_below, grid_b(n), dim3(256), 0, st, excl.as<int32_t>(), n,
+                           dense_frac * (double)nnz, Kd.as<int32_t>());
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(d2h(&K, Kd.p, 4, st));
+    }
+    const int64_t ndense = K > 0 ? ((int64_t)K + w - 1) / w : 0;
+    const int64_t nseg = ndense + ((int64_t)K < n || n == 0 ? 1 : 0);
+    l.nseg_dense = (int)ndense;
+    std::vector<int32_t> seg_pair;
+    {
+        TmpBuf ekey;
+        GBK_TRY(ekey.alloc(4 * z1));
+        if (nnz > 0) {
+            hipLaunchKernelGGL(k_lav_key, grid_b(nnz), dim3(256), 0, st, m.col, o2n.as<int32_t>(), nnz, (int64_t)K,
+                               w, ndense, ekey.as<uint32_t>());
+            GBK_LAUNCH_CHECK();
+        }
+        GBK_TRY(seg_pairs_device(m, type, nseg, ekey.as<uint32_t>(), o2n.as<int32_t>(), l.seg, seg_pair));
+    }
+    // merge-path tiles of the virtual CSR, ordered so a segment's tiles run on one XCD
+    SegCSR& s = l.seg;
+    GBK_TRY(build_merge_tiles(s.v, type));
+    std::vector<int2> tiles(s.v.ntiles + 1);
+    GBK_TRY(d2h(tiles.data(), s.v.tiles, sizeof(int2) * (s.v.ntiles + 1), st));
+    std::vector<int64_t> sp(seg_pair.begin(), seg_pair.end());
+    GBK_TRY(upload(&s.tile_map, xcd_tile_map(tiles, sp, (int)nseg)));
+    return GBK_SUCCESS;
+}
+
 static int build_lav(const DevCSR& m, int type, Lav** out) {
-    std::vector<int32_t> rp, col;
-    std::vector<char> val;
-    GBK_TRY(download_csr(m, type, rp, col, val));
-    const int64_t n = m.nrows;
-    std::vector<int32_t> o2n, n2o;
-    std::vector<int64_t> bounds;
-    gbk_host::lav_relabel(n, rp.data(), col.data(), 0.8, seg_cols_default(n), o2n, n2o, bounds);
-    gbk_host::SegHost h;
-    gbk_host::build_segmented(n, rp.data(), col.data(), val.data(), type_size(type), bounds, o2n.data(), h);
     Lav* l = new Lav();
-    int rc = build_seg_common(h, type, n, l->seg);
-    if (rc == GBK_SUCCESS) rc = upload(&l->new_to_old, n2o);
-    if (rc == GBK_SUCCESS) rc = dev_alloc(&l->xperm, type_size(type) * (n > 0 ? n : 1));
+    const int rc = build_lav_device(m, type, 0.8, seg_cols_default(m.nrows), *l);
     if (rc != GBK_SUCCESS) {
         free_seg_fields(l->seg);
         dev_free(l->new_to_old);
@@ -944,7 +1257,6 @@ static int build_lav(const DevCSR& m, int type, Lav** out) {
         delete l;
         return rc;
     }
-    l->nseg_dense = (int)bounds.size() - 1;
     *out = l;
     return GBK_SUCCESS;
 }

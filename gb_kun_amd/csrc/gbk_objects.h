@@ -158,6 +158,9 @@ int build_merge_tiles(DevCSR& m, int type);
 int bits_for(int64_t maxval);
 int dev_sort_pairs(const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t m, int bits,
                    hipStream_t st);
+int dev_sort_pairs64(const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout, int64_t m, int bits,
+                     hipStream_t st);
+int dev_exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t m, hipStream_t st);
 int dev_iota(int32_t* v, int64_t n, hipStream_t st);
 int dev_row_ids(const int32_t* rp, int64_t n, int32_t* rowid, hipStream_t st);
 int dev_count(const int32_t* key, int64_t m, int32_t* cnt, hipStream_t st);
