@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed loop, copy this many MiB device-to-device on the library stream "
                          "(a kernel of known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal", "io", "tc"],
+    ap.add_argument("--section", default="all", choices=["all", "batched", "served", "traversal", "io", "tc", "pr"],
                     help="run one secondary measurement alone (for profiling); prints that section as JSON")
     args = ap.parse_args()
 
@@ -84,6 +84,12 @@ def main():
             res = io_bench(g, args.scale, args.edge_factor)
         elif args.section == "tc":
             res = tc_bench(g, args.tc_scale)
+        elif args.section == "pr":
+            rp, col = g.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1, symmetric=False, permute=True)
+            fmt = {"csr": g.FMT_CSR, "sell_c": g.FMT_SELL_C, "csr_seg": g.FMT_CSR_SEG, "lav": g.FMT_LAV}.get(
+                args.format, g.FMT_CSR_SEG)
+            A = g.Matrix.from_csr(rp, col, g.inv_coldeg_f32(len(rp) - 1, col), fmt=fmt)
+            res = pr_bench(g, A, None)
         else:
             res = traversal_bench(g, args.bfs_scale)
         print(json.dumps({"section": args.section, args.section: res}), flush=True)
@@ -168,6 +174,8 @@ def main():
         g.synchronize()
         log(f"copy probe: {args.copy_probe} MiB device-to-device")
 
+    pr = pr_bench(g, A, ms)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(O, rp, col, vals, x_np, algo_bytes, args.cpu_seconds)
@@ -204,11 +212,38 @@ def main():
                      "kernel": KERNELS.get(fmt_name.rstrip("0123456789"), "SpMV"),
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
+        "pagerank": pr,
         "batched": batched,
         "served": served,
         "bfs_sssp": traversal,
     }
     print(json.dumps(out), flush=True)
+
+
+def pr_bench(g, A, spmv_ms):
+    """PageRank with sinks (damping 0.85) on the SpMV graph, ranks left in
+    HBM: a fixed 20 iterations (time per iteration beside one SpMV's), then
+    to L1 < 1e-6."""
+    from gb_kun_amd.device import DeviceArray
+    r = DeviceArray.zeros(A.n, np.float32)
+    g.pagerank_device(A, r.ptr, itermax=2, tol=0.0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.pagerank_device(A, r.ptr, itermax=20, tol=0.0)
+    t_fixed = (time.perf_counter() - t0) / 20
+    t0 = time.perf_counter()
+    iters = g.pagerank_device(A, r.ptr, itermax=200, tol=1e-6)
+    t_tol = time.perf_counter() - t0
+    res = {"ms_per_iteration": round(t_fixed * 1e3, 4), "tol": 1e-6, "iterations_to_tol": int(iters),
+           "ms_to_tol": round(t_tol * 1e3, 3), "ms_per_iteration_to_tol": round(t_tol * 1e3 / max(iters, 1), 4)}
+    msg = (f"PageRank: {t_fixed * 1e3:.3f} ms/iteration (20 fixed); L1 < 1e-6 in {iters} iterations, "
+           f"{t_tol * 1e3:.1f} ms")
+    if spmv_ms:
+        res["spmv_ms"] = round(spmv_ms, 4)
+        res["iteration_over_spmv"] = round(t_fixed * 1e3 / spmv_ms, 3)
+        msg += f"; iteration / SpMV = {t_fixed * 1e3 / spmv_ms:.2f}"
+    log(msg)
+    return res
 
 
 def _sources(rp, k, seed):

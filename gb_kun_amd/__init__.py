@@ -137,6 +137,7 @@ _sig("gbk_sssp_device", C.c_int, _vp, _i64, _vp, _pi64)
 _sig("gbk_sssp", C.c_int, _vp, _i64, _vp, _pi64)
 _sig("gbk_cc", C.c_int, _vp, _pi32, _pi64)
 _sig("gbk_pagerank", C.c_int, _vp, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
+_sig("gbk_pagerank_device", C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
 _sig("gbk_spmm_device", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, Semiring, C.c_int)
 _sig("gbk_msbfs", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
 _sig("gbk_msbfs_device", C.c_int, _vp, _pi64, C.c_int, _vp, _pi64)
@@ -522,6 +523,13 @@ def pagerank(A, itermax=100, damping=0.85, tol=0.0):
     it = C.c_int(0)
     check(lib.gbk_pagerank(A.h, _ptr(r, C.c_float), itermax, damping, tol, C.byref(it)))
     return r, it.value
+
+
+def pagerank_device(A, ranks_ptr, itermax=100, damping=0.85, tol=0.0):
+    """PageRank writing n float ranks to a device array; returns the iterations run."""
+    it = C.c_int(0)
+    check(lib.gbk_pagerank_device(A.h, _vp(ranks_ptr), itermax, damping, tol, C.byref(it)))
+    return it.value
 
 
 # ------------------------------------------------------------- batched ops

@@ -561,76 +561,169 @@ static int cc_run(gbk_matrix A, int32_t* parent, int64_t* stats) {
 }
 
 // -------------------------------------------------------------------- PR --
-// w = t ./ d with d = max(deg / damping, 1 / damping); also sums the ranks of
-// sinks (deg == 0) into *sink.
-__global__ __launch_bounds__(256) void k_pr_prep(const int32_t* __restrict__ crp, const float* __restrict__ t,
-                                                 float* __restrict__ w, float* __restrict__ r, int64_t n, float damping,
-                                                 double* __restrict__ sink) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0;
-    if (i < n) {
-        const int deg = crp[i + 1] - crp[i];
-        float d = (float)deg / damping;
-        const float dmin = 1.0f / damping;
-        if (d < dmin) d = dmin;
-        const float ti = t[i];
-        w[i] = ti / d;
-        if (deg == 0) s = ti;
+// PageRank with sinks (LAGraph_page_rank_sinks, algorithms/pr/pr.hpp:20-153).
+// Per iteration: one SpMV r' = A (+.second) w written without accumulation,
+// then k_pr_post adds the teleport term (1 - damping)/n + damping/n * (sum of
+// the sinks' ranks) to every row, accumulates the L1 change, and forms the
+// next iteration's w = r' ./ d and sink sum.  The reference's t = r copy,
+// w = t ./ d pass, teleport assign, accumulate and diff reduction are this one
+// pass; the ranks alternate between two buffers.  The teleport term is added
+// to the finished row sum (float addition commutes), so the bits equal
+// "r = teleport; r += A w".  d[i] = max(deg / damping, 1 / damping) is kept
+// negated for sinks (deg == 0).
+
+// The per-vertex sums (sink ranks, L1 change) are reduced per block into
+// partials and then by one block in a fixed order: no atomics on one address
+// (65 K waves adding into one double serialize for most of a millisecond at
+// n = 4 M) and the same sums on every run.
+constexpr int PR_THREADS = 256;
+constexpr int PR_MAX_BLOCKS = 2048;
+
+static int pr_blocks(int64_t n) {
+    const int64_t b = (n + PR_THREADS - 1) / PR_THREADS;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, PR_MAX_BLOCKS));
+}
+
+// Block sum of (a, b) into out[0], out[1] (thread 0 writes).
+__device__ __forceinline__ void pr_block_sum2(double a, double b, double* __restrict__ out) {
+    __shared__ double sa[PR_THREADS / 64], sb[PR_THREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(sink, s);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sa[wv] = a;
+        sb[wv] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+        for (int i = 0; i < PR_THREADS / 64; ++i) {
+            ta += sa[i];
+            tb += sb[i];
+        }
+        out[0] = ta;
+        out[1] = tb;
+    }
 }
 
-__global__ void k_pr_fill(float* __restrict__ r, int64_t n, const double* __restrict__ sink, double base, double coef) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) r[i] = (float)(base + coef * *sink);
-}
-
-__global__ __launch_bounds__(256) void k_pr_diff(const float* __restrict__ t, const float* __restrict__ r, int64_t n,
-                                                 double* __restrict__ diff) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(PR_THREADS) void k_pr_init(const int32_t* __restrict__ crp, int64_t n, float damping,
+                                                        float r0, float* __restrict__ r, float* __restrict__ w,
+                                                        float* __restrict__ d, double* __restrict__ part) {
     double s = 0.0;
-    if (i < n) s = fabs((double)t[i] - (double)r[i]);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(diff, s);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int deg = crp[i + 1] - crp[i];
+        float di = (float)deg / damping;
+        const float dmin = 1.0f / damping;
+        if (di < dmin) di = dmin;
+        d[i] = deg == 0 ? -di : di;
+        r[i] = r0;
+        w[i] = r0 / di;
+        if (deg == 0) s += r0;
+    }
+    pr_block_sum2(s, 0.0, part + 2 * blockIdx.x);
 }
 
+__global__ __launch_bounds__(PR_THREADS) void k_pr_post(const float* __restrict__ rold, float* __restrict__ rnew,
+                                                        const float* __restrict__ d, float* __restrict__ w, int64_t n,
+                                                        const double* __restrict__ sink_in, double base, double coef,
+                                                        double* __restrict__ part) {
+    const float tele = (float)(base + coef * *sink_in);
+    double s = 0.0, df = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float r = tele + rnew[i];
+        rnew[i] = r;
+        df += fabs((double)rold[i] - (double)r);
+        const float di = d[i];
+        w[i] = r / fabsf(di);
+        if (di < 0.0f) s += r;
+    }
+    pr_block_sum2(s, df, part + 2 * blockIdx.x);
+}
+
+// out[0], out[1] = sums of the nb block partials (one block, fixed order).
+__global__ __launch_bounds__(PR_THREADS) void k_pr_final(const double* __restrict__ part, int nb,
+                                                         double* __restrict__ out) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += PR_THREADS) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    pr_block_sum2(a, b, out);
+}
+
+// A small pinned host buffer per thread for the L1 changes, so that reading
+// one does not wait for work queued after it.
+static double* pinned_diff() {
+    thread_local double* p = nullptr;
+    if (!p && hipHostMalloc((void**)&p, 64, hipHostMallocDefault) != hipSuccess) p = nullptr;
+    return p;
+}
+
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~EventPair() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+// With tol > 0 the host reads iteration k's L1 change while iteration k + 1
+// runs; when it is below tol the result of iteration k + 1 is dropped, so the
+// ranks and the iteration count equal those of a loop that waits each time.
 static int pr_run(gbk_matrix A, float* ranks_dev, int itermax, double damping, double tol, int* iters_out) {
     if (A->type != GBK_FP32) return fail(GBK_DOMAIN_MISMATCH, "PageRank needs an FP32 matrix");
     const int64_t n = A->n;
     hipStream_t st = stream();
-    float* t = (float*)A->ws.buf[0];
+    float* R[2] = {ranks_dev, (float*)A->ws.buf[0]};
     float* w = (float*)A->ws.buf[1];
+    float* d = w + n;  // ws buffers hold n * 8 bytes
+    const int nb = pr_blocks(n);
     void* p;
-    GBK_TRY(scratch(6, 64, &p));
-    double* acc = (double*)p;  // [0] sink sum, [1] L1 diff
-    float* r = ranks_dev;
-    GBK_TRY(fill_value(GBK_FP32, r, n, 1.0 / (double)n));
-    const DevCSR& csc = A->csc;
-    int it = 0;
-    for (it = 0; it < itermax;) {
-        // t = r (old ranks); w = t ./ d; sink sum
-        GBK_HIP(hipMemcpyAsync(t, r, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-        GBK_TRY(fill_value(GBK_FP64, acc, 2, 0.0));
-        hipLaunchKernelGGL(k_pr_prep, grid_n(n), dim3(256), 0, st, csc.row_ptr, t, w, r, n, (float)damping, acc);
+    GBK_TRY(scratch(6, 64 + 16 * (size_t)nb, &p));
+    double* acc = (double*)p;  // slot k & 1: [sink sum, L1 change] of iteration k
+    double* part = acc + 8;    // block partials
+    double* hd = nullptr;
+    EventPair ev;
+    if (tol > 0) {
+        hd = pinned_diff();
+        if (!hd) return fail(GBK_OUT_OF_MEMORY, "pinned host buffer");
+        for (auto& e : ev.e) GBK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    hipLaunchKernelGGL(k_pr_init, dim3(nb), dim3(PR_THREADS), 0, st, A->csc.row_ptr, n, (float)damping,
+                       (float)(1.0 / (double)n), R[0], w, d, part);
+    GBK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pr_final, dim3(1), dim3(PR_THREADS), 0, st, part, nb, acc);
+    GBK_LAUNCH_CHECK();
+    const double base = (1.0 - damping) / (double)n, coef = damping / (double)n;
+    int res = itermax;
+    for (int k = 1; k <= itermax; ++k) {
+        double* out = acc + 2 * (k & 1);
+        GBK_TRY(spmv_unmasked(A, 0, w, R[k & 1], GBK_ADD_PLUS, GBK_MUL_SECOND, GBK_ACC_NONE));
+        hipLaunchKernelGGL(k_pr_post, dim3(nb), dim3(PR_THREADS), 0, st, R[(k - 1) & 1], R[k & 1], d, w, n,
+                           acc + 2 * ((k - 1) & 1), base, coef, part);
         GBK_LAUNCH_CHECK();
-        // r = teleport; r += A (+.second) w
-        hipLaunchKernelGGL(k_pr_fill, grid_n(n), dim3(256), 0, st, r, n, acc, (1.0 - damping) / (double)n,
-                           damping / (double)n);
+        hipLaunchKernelGGL(k_pr_final, dim3(1), dim3(PR_THREADS), 0, st, part, nb, out);
         GBK_LAUNCH_CHECK();
-        GBK_TRY(spmv_unmasked(A, 0, w, r, GBK_ADD_PLUS, GBK_MUL_SECOND, GBK_ACC_PLUS));
-        ++it;
         if (tol > 0) {
-            hipLaunchKernelGGL(k_pr_diff, grid_n(n), dim3(256), 0, st, t, r, n, acc + 1);
-            GBK_LAUNCH_CHECK();
-            double d = 0;
-            GBK_HIP(hipMemcpyAsync(&d, acc + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-            GBK_HIP(hipStreamSynchronize(st));
-            if (d < tol) break;
+            GBK_HIP(hipMemcpyAsync(hd + (k & 1), out + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+            GBK_HIP(hipEventRecord(ev.e[k & 1], st));
+            if (k >= 2) {
+                GBK_HIP(hipEventSynchronize(ev.e[(k - 1) & 1]));
+                if (hd[(k - 1) & 1] < tol) {
+                    res = k - 1;
+                    break;
+                }
+            }
         }
     }
     GBK_HIP(hipStreamSynchronize(st));
-    if (iters_out) *iters_out = it;
+    if (res & 1) GBK_HIP(hipMemcpyAsync(R[0], R[1], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    if (iters_out) *iters_out = res;
     return GBK_SUCCESS;
 }
 
@@ -693,6 +786,16 @@ extern "C" int gbk_cc(gbk_matrix A, int32_t* labels, int64_t* stats) {
     GBK_HIP(hipMemcpyAsync(labels, p, sizeof(int32_t) * A->n, hipMemcpyDeviceToHost, stream()));
     GBK_HIP(hipStreamSynchronize(stream()));
     return GBK_SUCCESS;
+}
+
+extern "C" int gbk_pagerank_device(gbk_matrix A, float* ranks_dev, int itermax, double damping, double tol,
+                                   int* iters) {
+    GBK_TRY(check_graph(A));
+    if (!ranks_dev) return fail(GBK_NULL_POINTER, "null ranks");
+    if (itermax < 0 || damping <= 0 || damping >= 1) return fail(GBK_INVALID_VALUE, "bad PageRank parameters");
+    if (ranks_dev == (float*)A->ws.buf[0] || ranks_dev == (float*)A->ws.buf[1])
+        return fail(GBK_INVALID_VALUE, "ranks may not live in the matrix workspace");
+    return pr_run(A, ranks_dev, itermax, damping, tol, iters);
 }
 
 extern "C" int gbk_pagerank(gbk_matrix A, float* ranks, int itermax, double damping, double tol, int* iters) {

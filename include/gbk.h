@@ -383,6 +383,10 @@ int gbk_cc(gbk_matrix A, int32_t* labels, int64_t* stats);
  * ranks (host float[n]).  iters receives the iterations run. */
 int gbk_pagerank(gbk_matrix A, float* ranks, int itermax, double damping, double tol, int* iters);
 
+/* The same with the ranks left in a device array of n floats (not the
+ * matrix's own workspace). */
+int gbk_pagerank_device(gbk_matrix A, float* ranks_dev, int itermax, double damping, double tol, int* iters);
+
 /* ------------------------------------------------------------------------ */
 /* Batched (multi-vector) operations.  The reference has no batched form: each
  * is the k-fold of one reference call, computed in one pass over the matrix. */

@@ -229,6 +229,36 @@ def test_pagerank_vs_fp64(gbk, oracle, scale):
     assert abs(r.astype(np.float64).sum() - 1.0) < 1e-5
 
 
+def test_pagerank_tol_stop_equals_fixed_run(gbk):
+    """With tol > 0 the host reads iteration k's L1 change while iteration
+    k + 1 runs and drops k + 1 once k is below tol: the ranks and the count
+    equal a fixed run of that many iterations, whose L1 change is the first
+    below tol.  Odd and even counts (the result in either rank buffer), zero
+    iterations and the device entry point give the same bits."""
+    from gb_kun_amd.device import DeviceArray
+    rp, col = gbk.gen_rmat(12, 16, seed=101)
+    n = len(rp) - 1
+    A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.float32))
+    r, it = gbk.pagerank(A, itermax=200, tol=1e-6)
+    assert 3 < it < 200
+    r_fixed, it_fixed = gbk.pagerank(A, itermax=it, tol=0.0)
+    assert it_fixed == it and np.array_equal(r, r_fixed)
+    r_prev, _ = gbk.pagerank(A, itermax=it - 1)
+    r_prev2, _ = gbk.pagerank(A, itermax=it - 2)
+    l1 = np.abs(r.astype(np.float64) - r_prev).sum()
+    l1_prev = np.abs(r_prev.astype(np.float64) - r_prev2).sum()
+    assert l1 < 1e-6 <= l1_prev
+    r0, it0 = gbk.pagerank(A, itermax=0)
+    assert it0 == 0 and np.all(r0 == np.float32(1.0 / n))
+    d = DeviceArray.zeros(n, np.float32)
+    for k in (1, 2, 3):
+        rh, _ = gbk.pagerank(A, itermax=k)
+        assert gbk.pagerank_device(A, d.ptr, itermax=k) == k
+        assert np.array_equal(d.numpy(), rh), k
+    assert gbk.pagerank_device(A, d.ptr, itermax=200, tol=1e-6) == it
+    assert np.array_equal(d.numpy(), r)
+
+
 def test_bfs_pull_heavy_vertex(gbk, oracle):
     """Two hubs over the same 5000 leaves: the pull steps meet an unvisited hub
     with more in-edges than the per-group limit (the one-workgroup-per-vertex
