@@ -31,7 +31,8 @@ METRIC = "GTEPS & HBM GB/s (% peak), semiring SpMV/BFS RMAT-22; batched/served q
 HBM_PEAK_GBS = 8000.0
 KERNELS = {
     "csr": "k_spmv_merge + k_spmv_merge_fixup",
-    "csr_seg": "k_hot_x + k_seg_hot (persistent, XCD-mapped segments, hot x in LDS) + k_hot_fix + k_seg_merge",
+    "csr_seg": "k_hot_x + k_seg_flag (persistent, XCD-mapped segments, hot x in LDS, flagged pair ends) + "
+               "k_flag_fix + k_seg_merge8",
     "lav": "k_gather + k_spmv_merge (XCD-mapped segment tiles) + k_spmv_merge_fixup + k_seg_merge",
     "sell_c": "k_sell_c + long-row k_spmv_merge + k_spmv_merge_fixup",
 }

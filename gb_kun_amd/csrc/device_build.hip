@@ -28,16 +28,40 @@ __global__ void k_iota(int32_t* __restrict__ v, int64_t n) {
     if (i < n) v[i] = (int32_t)i;
 }
 
-// rowid[e] = r for rp[r] <= e < rp[r + 1]; one thread per row.
-__global__ void k_row_ids(const int32_t* __restrict__ rp, int64_t n, int32_t* __restrict__ rowid) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    for (int32_t e = rp[r]; e < rp[r + 1]; ++e) rowid[e] = (int32_t)r;
+// rowid[e] = r for rp[r] <= e < rp[r + 1].  A workgroup takes 256
+// consecutive rows, whose entries are contiguous, and walks those entries
+// together (each finds its row by a binary search over the row pointers in
+// LDS), so a row of many entries is not left to one thread.
+__global__ __launch_bounds__(256) void k_row_ids(const int32_t* __restrict__ rp, int64_t n,
+                                                 int32_t* __restrict__ rowid) {
+    __shared__ int32_t srp[257];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int rows = (int)min((int64_t)256, n - r0);
+    for (int i = threadIdx.x; i <= rows; i += 256) srp[i] = rp[r0 + i];
+    __syncthreads();
+    const int32_t e1 = srp[rows];
+    for (int32_t e = srp[0] + (int32_t)threadIdx.x; e < e1; e += 256) {
+        int lo = 0, hi = rows - 1;  // the last row starting at or before e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (srp[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        rowid[e] = (int32_t)(r0 + lo);
+    }
 }
 
-__global__ void k_count(const int32_t* __restrict__ key, int64_t m, int32_t* __restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) atomicAdd(cnt + key[i], 1);
+// ptr[b] = first i with key[i] >= b (keys ascending), b in [0, nbins].
+__global__ void k_lower_bounds(const uint32_t* __restrict__ key, int64_t m, int64_t nbins, int32_t* __restrict__ ptr) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nbins) return;
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)key[mid] < b) lo = mid + 1;
+        else hi = mid;
+    }
+    ptr[b] = (int32_t)lo;
 }
 
 __global__ void k_gather_i32(const int32_t* __restrict__ src, const int32_t* __restrict__ idx, int64_t m,
@@ -131,10 +155,11 @@ int dev_row_ids(const int32_t* rp, int64_t n, int32_t* rowid, hipStream_t st) {
     return GBK_SUCCESS;
 }
 
-// cnt[key[i]] += 1 (cnt zeroed by the caller).
-int dev_count(const int32_t* key, int64_t m, int32_t* cnt, hipStream_t st) {
-    if (m <= 0) return GBK_SUCCESS;
-    hipLaunchKernelGGL(k_count, grid_n(m), dim3(256), 0, st, key, m, cnt);
+// Bucket pointers of ascending keys: ptr[b] = number of keys below b for b in
+// [0, nbins] (one binary search per bucket: no atomics, so heavily repeated
+// keys cost nothing extra).
+int dev_bounds_sorted(const uint32_t* key, int64_t m, int64_t nbins, int32_t* ptr, hipStream_t st) {
+    hipLaunchKernelGGL(k_lower_bounds, grid_n(nbins + 1), dim3(256), 0, st, key, m, nbins, ptr);
     GBK_LAUNCH_CHECK();
     return GBK_SUCCESS;
 }
@@ -174,27 +199,19 @@ int device_transpose(const DevCSR& in, int type, int64_t n, DevCSR& out) {
     GBK_TRY(dev_alloc((void**)&out.row_ptr, sizeof(int32_t) * (n + 1)));
     GBK_TRY(dev_alloc((void**)&out.col, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
     GBK_TRY(dev_alloc(&out.val, vs * (nnz > 0 ? nnz : 1)));
-    // column counts -> column pointers
-    int32_t* cnt = nullptr;
-    GBK_TRY(dev_alloc((void**)&cnt, sizeof(int32_t) * (n + 1)));
-    GBK_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (n + 1), st));
-    GBK_TRY(dev_count(in.col, nnz, cnt, st));
-    int32_t total = 0;
-    // writes row_ptr[0..n], row_ptr[n] = total
-    int rc = exclusive_scan_i32(cnt, out.row_ptr, n, &total);
-    dev_free(cnt);
-    GBK_TRY(rc);
-    // stable sort of entry ids by column: rows ascend inside every column
+    // stable sort of entry ids by column: rows ascend inside every column;
+    // the column pointers are the bucket bounds of the sorted columns
     int32_t *perm_in = nullptr, *perm = nullptr, *rowid = nullptr;
     uint32_t* keys_out = nullptr;
     const int64_t m1 = nnz > 0 ? nnz : 1;
-    rc = dev_alloc((void**)&perm_in, sizeof(int32_t) * m1);
+    int rc = dev_alloc((void**)&perm_in, sizeof(int32_t) * m1);
     if (rc == GBK_SUCCESS) rc = dev_alloc((void**)&perm, sizeof(int32_t) * m1);
     if (rc == GBK_SUCCESS) rc = dev_alloc((void**)&keys_out, sizeof(uint32_t) * m1);
     if (rc == GBK_SUCCESS) rc = dev_alloc((void**)&rowid, sizeof(int32_t) * m1);
     if (rc == GBK_SUCCESS) rc = dev_iota(perm_in, nnz, st);
     if (rc == GBK_SUCCESS)
         rc = dev_sort_pairs((const uint32_t*)in.col, keys_out, perm_in, perm, nnz, bits_for(n > 0 ? n - 1 : 0), st);
+    if (rc == GBK_SUCCESS) rc = dev_bounds_sorted(keys_out, nnz, n, out.row_ptr, st);
     if (rc == GBK_SUCCESS) rc = dev_row_ids(in.row_ptr, n, rowid, st);
     if (rc == GBK_SUCCESS) rc = dev_gather_i32(rowid, perm, nnz, out.col, st);
     if (rc == GBK_SUCCESS) rc = dev_gather_bytes(in.val, perm, nnz, vs, out.val, st);

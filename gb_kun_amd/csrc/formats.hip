@@ -50,14 +50,19 @@ struct SellC {
 // rest read the segment's x slice through that XCD's L2.
 struct SegHot {
     int H = 0;                    // LDS slots per segment
-    int64_t ntiles = 0;           // wave tiles (HOT_WT merge items each)
-    int32_t* hcol = nullptr;      // entries, hot columns as -1 - slot
+    int32_t* hcol = nullptr;      // during the build: entries, hot columns as -1 - slot
     int32_t* hot_ids = nullptr;   // nseg * H original columns (-1: unused slot)
     void* xh = nullptr;           // nseg * H gathered x values, per call
-    int2* tiles = nullptr;        // ntiles + 1
-    int64_t* seg_tile = nullptr;  // nseg + 1
-    void* carry = nullptr;        // 2 * ntiles values
     int32_t* seg_pair = nullptr;  // nseg + 1: first pair of each segment
+    // flagged-entry layout (k_seg_flag): fcol[e] = (e ends its pair) << 31 |
+    // (hot ? 1 << 30 | slot : column), padded with zeros to whole 512-entry
+    // chunks; a tile is one chunk's entries of one segment, ftiles[t] =
+    // {first entry, first pair | (that pair began in an earlier tile) << 31}.
+    int32_t* fcol = nullptr;
+    int64_t nftiles = 0;
+    int2* ftiles = nullptr;        // nftiles + 1 (the last = {nnz, 0})
+    int64_t* fseg_tile = nullptr;  // nseg + 1
+    void* fcarry = nullptr;        // 2 * nftiles values
 };
 
 struct SegCSR {
@@ -69,7 +74,7 @@ struct SegCSR {
     int32_t* row_pairs = nullptr;
     void* partial = nullptr;
     int64_t n = 0;
-    SegHot hot;  // built for CSR_SEG (hot.tiles == nullptr otherwise)
+    SegHot hot;  // built for CSR_SEG (hot.fcol == nullptr otherwise)
 };
 
 struct Lav {
@@ -487,10 +492,11 @@ static void free_seg_fields(SegCSR& s) {
     dev_free(s.hot.hcol);
     dev_free(s.hot.hot_ids);
     dev_free(s.hot.xh);
-    dev_free(s.hot.tiles);
-    dev_free(s.hot.seg_tile);
-    dev_free(s.hot.carry);
     dev_free(s.hot.seg_pair);
+    dev_free(s.hot.fcol);
+    dev_free(s.hot.ftiles);
+    dev_free(s.hot.fseg_tile);
+    dev_free(s.hot.fcarry);
     s = SegCSR();
 }
 
@@ -532,19 +538,14 @@ static int64_t seg_cols_default(int64_t n) {
     return 512 * 1024;
 }
 
-// Persistent hot-column kernel shape: 16 waves per block, each wave merges
-// tiles of HOT_WT items (8 per lane) on its own LDS staging area (one
-// item-kind byte per item and HOT_WT products).
+// Persistent hot-column kernel shape: 16 waves per block, the block's LDS
+// holding the hot x values of its segment.
 constexpr int HOT_WAVES = 16;
-constexpr int HOT_WT = 512;
-
-static size_t hot_staging_bytes(size_t vs) { return (size_t)HOT_WAVES * (HOT_WT + HOT_WT * vs); }
 
 // LDS slots per segment for values of vs bytes: what one workgroup's LDS
-// holds beside the staging areas, a multiple of 64.
+// holds, a multiple of 64.
 static int hot_slots(size_t vs) {
-    const int64_t lds = device_lds_bytes();
-    const int64_t room = lds - (int64_t)hot_staging_bytes(vs);
+    const int64_t room = device_lds_bytes();
     if (room <= 0) return 0;
     int64_t h = (room / (int64_t)vs) & ~(int64_t)63;
     return h < 1024 ? 0 : (int)std::min<int64_t>(h, 1 << 20);
@@ -582,6 +583,12 @@ __global__ void k_pair_fill(const int32_t* __restrict__ flag, const int32_t* __r
     vrp[p] = (int32_t)i;
     pair_row[p] = rowid[perm[i]];
     pair_seg[p] = (int32_t)skey[i];
+}
+
+// deg[c] = entries of column c = row length of c in the other orientation.
+__global__ void k_deg_from_ptr(const int32_t* __restrict__ optr, int64_t n, int32_t* __restrict__ deg) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) deg[c] = optr[c + 1] - optr[c];
 }
 
 __global__ void k_set_i32(int32_t* __restrict__ p, int64_t i, int32_t v) {
@@ -625,35 +632,6 @@ __global__ void k_hot_encode(const int32_t* __restrict__ vcol, const int32_t* __
     const int32_t c = vcol[i];
     const int32_t sl = slot[c];
     hcol[i] = sl >= 0 ? -1 - sl : c;
-}
-
-// Merge-path start of every wave tile: tile t of segment s starts at
-// diagonal (t - seg_tile[s]) * HOT_WT of the segment's pairs [p0, p1).
-__global__ void k_seg_tiles(const int32_t* __restrict__ vrp, const int64_t* __restrict__ seg_tile,
-                            const int32_t* __restrict__ seg_pair, int nseg, int64_t ntiles, int64_t npairs,
-                            int64_t nnz, int2* __restrict__ tiles) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > ntiles) return;
-    if (t == ntiles) {
-        tiles[t] = make_int2((int)npairs, (int)nnz);
-        return;
-    }
-    int lo_s = 0, hi_s = nseg - 1;  // last segment with seg_tile[s] <= t
-    while (lo_s < hi_s) {
-        const int mid = (lo_s + hi_s + 1) >> 1;
-        if (seg_tile[mid] <= t) lo_s = mid;
-        else hi_s = mid - 1;
-    }
-    const int64_t p0 = seg_pair[lo_s], p1 = seg_pair[lo_s + 1];
-    const int64_t k_lo = vrp[p0], nr = p1 - p0, nk = (int64_t)vrp[p1] - k_lo;
-    const int64_t diag = min((t - seg_tile[lo_s]) * (int64_t)HOT_WT, nr + nk);
-    int64_t lo = max(diag - nk, (int64_t)0), hi = min(diag, nr);
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)vrp[p0 + mid + 1] - k_lo <= diag - mid - 1) lo = mid + 1;
-        else hi = mid;
-    }
-    tiles[t] = make_int2((int)(p0 + lo), (int)(k_lo + diag - lo));
 }
 
 static dim3 grid_e(int64_t n) {
@@ -714,7 +692,11 @@ static int seg_pairs_device(const DevCSR& m, int type, int64_t nseg, const uint3
     s.v.nnz = nnz;
     GBK_TRY(dev_alloc((void**)&s.v.row_ptr, 4 * (npairs + 1)));
     GBK_TRY(dev_alloc((void**)&s.v.col, 4 * z1));
-    GBK_TRY(dev_alloc(&s.v.val, vs * z1));
+    // values padded with zeros to whole 512-entry chunks (the flagged
+    // kernel loads whole chunks)
+    const int64_t zpad = (nnz + 511) / 512 * 512 + 512;
+    GBK_TRY(dev_alloc(&s.v.val, vs * zpad));
+    GBK_HIP(hipMemsetAsync((char*)s.v.val + vs * nnz, 0, vs * (zpad - nnz), st));
     if (nnz > 0) {
         hipLaunchKernelGGL(k_pair_fill, grid_e(nnz), dim3(256), 0, st, flag.as<int32_t>(), pidx.as<int32_t>(),
                            skey.as<uint32_t>(), perm.as<int32_t>(), rowid.as<int32_t>(), nnz, s.v.row_ptr, pair_row,
@@ -729,16 +711,12 @@ static int seg_pairs_device(const DevCSR& m, int type, int64_t nseg, const uint3
         GBK_LAUNCH_CHECK();
     }
     GBK_TRY(dev_gather_bytes(m.val, perm.as<int32_t>(), nnz, vs, s.v.val, st));
-    // first pair of every segment
-    TmpBuf segcnt;
-    GBK_TRY(segcnt.alloc(4 * (nseg + 1)));
-    GBK_HIP(hipMemsetAsync(segcnt.p, 0, 4 * (nseg + 1), st));
-    GBK_TRY(dev_count(pair_seg.as<int32_t>(), npairs, segcnt.as<int32_t>(), st));
-    std::vector<int32_t> sc(nseg + 1);
-    GBK_HIP(hipMemcpyAsync(sc.data(), segcnt.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, st));
-    GBK_HIP(hipStreamSynchronize(st));
+    // first pair of every segment (pairs are in segment order)
+    TmpBuf segptr;
+    GBK_TRY(segptr.alloc(4 * (nseg + 1)));
+    GBK_TRY(dev_bounds_sorted((const uint32_t*)pair_seg.p, npairs, nseg, segptr.as<int32_t>(), st));
     seg_pair.assign(nseg + 1, 0);
-    for (int64_t g = 0; g < nseg; ++g) seg_pair[g + 1] = seg_pair[g] + sc[g];
+    GBK_TRY(d2h(seg_pair.data(), segptr.p, 4 * (nseg + 1), st));
     // rows -> their pairs in segment order: pairs stably sorted by row
     TmpBuf piota, pkey_out;
     GBK_TRY(piota.alloc(4 * (npairs > 0 ? npairs : 1)));
@@ -748,20 +726,104 @@ static int seg_pairs_device(const DevCSR& m, int type, int64_t nseg, const uint3
     GBK_TRY(dev_iota(piota.as<int32_t>(), npairs, st));
     GBK_TRY(dev_sort_pairs((const uint32_t*)pair_row, pkey_out.as<uint32_t>(), piota.as<int32_t>(), s.row_pairs,
                            npairs, bits_for(n > 0 ? n - 1 : 0), st));
-    {
-        TmpBuf rcnt;
-        GBK_TRY(rcnt.alloc(4 * (n + 1)));
-        GBK_HIP(hipMemsetAsync(rcnt.p, 0, 4 * (n + 1), st));
-        GBK_TRY(dev_count(pair_row, npairs, rcnt.as<int32_t>(), st));
-        int32_t tot = 0;
-        GBK_TRY(exclusive_scan_i32(rcnt.as<int32_t>(), s.rpp, n, &tot));  // rpp[0..n]
-    }
+    GBK_TRY(dev_bounds_sorted(pkey_out.as<uint32_t>(), npairs, n, s.rpp, st));  // rpp[0..n]
     GBK_TRY(dev_alloc(&s.partial, vs * (npairs > 0 ? npairs : 1)));
     GBK_HIP(hipStreamSynchronize(st));
     return GBK_SUCCESS;
 }
 
-static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, SegCSR& s) {
+// Merge-path tiles of the virtual CSR, ordered so that a segment's tiles run
+// on one XCD (the kernel of the formats without hot columns).
+static int seg_plain_tiles(SegCSR& s, int type, const std::vector<int32_t>& seg_pair) {
+    GBK_TRY(build_merge_tiles(s.v, type));
+    std::vector<int2> tiles(s.v.ntiles + 1);
+    GBK_TRY(d2h(tiles.data(), s.v.tiles, sizeof(int2) * (s.v.ntiles + 1), stream()));
+    std::vector<int64_t> sp(seg_pair.begin(), seg_pair.end());
+    return upload(&s.tile_map, xcd_tile_map(tiles, sp, s.nseg));
+}
+
+constexpr int FLAG_CHUNK = 512;  // entries per flagged tile (8 per lane of a wave)
+
+__global__ void k_flag_encode(const int32_t* __restrict__ hcol, int64_t nnz, int64_t padded,
+                              int32_t* __restrict__ fcol) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= padded) return;
+    if (e >= nnz) {
+        fcol[e] = 0;  // padding: column 0, never counted
+        return;
+    }
+    const int32_t c = hcol[e];
+    fcol[e] = c < 0 ? (int32_t)(0x40000000u | (uint32_t)(-1 - c)) : c;
+}
+
+__global__ void k_flag_ends(const int32_t* __restrict__ vrp, int64_t npairs, int32_t* __restrict__ fcol) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < npairs) fcol[vrp[p + 1] - 1] |= (int32_t)0x80000000u;
+}
+
+// Tile t: the part of chunk c in segment s, with its first pair (the pair
+// holding its first entry) and whether that pair began before the tile.
+__global__ void k_flag_tiles(const int32_t* __restrict__ vrp, const int64_t* __restrict__ fseg_tile,
+                             const int32_t* __restrict__ seg_pair, int nseg, int64_t ntiles, int64_t nnz,
+                             int2* __restrict__ ftiles) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    if (t == ntiles) {
+        ftiles[t] = make_int2((int)nnz, 0);
+        return;
+    }
+    int lo_s = 0, hi_s = nseg - 1;  // last segment with fseg_tile[s] <= t
+    while (lo_s < hi_s) {
+        const int mid = (lo_s + hi_s + 1) >> 1;
+        if (fseg_tile[mid] <= t) lo_s = mid;
+        else hi_s = mid - 1;
+    }
+    const int64_t p0 = seg_pair[lo_s], p1 = seg_pair[lo_s + 1];
+    const int64_t sk = vrp[p0];
+    const int64_t k0 = max(sk, (sk / FLAG_CHUNK + (t - fseg_tile[lo_s])) * FLAG_CHUNK);
+    int64_t lo = p0, hi = p1 - 1;  // last pair of the segment starting at or before k0
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (vrp[mid] <= k0) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t defer = vrp[lo] < k0 ? 0x80000000u : 0u;
+    ftiles[t] = make_int2((int)k0, (int)((uint32_t)lo | defer));
+}
+
+// The flagged layout from the hot encoding, the virtual row pointers and the
+// segments' pair / entry bounds.
+static int build_flagged(SegCSR& s, size_t vs, const std::vector<int32_t>& seg_pair,
+                         const std::vector<int32_t>& seg_k) {
+    hipStream_t st = stream();
+    SegHot& o = s.hot;
+    const int64_t nnz = s.v.nnz, nseg = s.nseg;
+    const int64_t padded = (nnz + FLAG_CHUNK - 1) / FLAG_CHUNK * FLAG_CHUNK + FLAG_CHUNK;
+    GBK_TRY(dev_alloc((void**)&o.fcol, 4 * padded));
+    hipLaunchKernelGGL(k_flag_encode, grid_e(padded), dim3(256), 0, st, o.hcol, nnz, padded, o.fcol);
+    GBK_LAUNCH_CHECK();
+    if (s.npairs > 0) {
+        hipLaunchKernelGGL(k_flag_ends, grid_e(s.npairs), dim3(256), 0, st, s.v.row_ptr, s.npairs, o.fcol);
+        GBK_LAUNCH_CHECK();
+    }
+    std::vector<int64_t> fseg_tile(nseg + 1, 0);
+    for (int64_t g = 0; g < nseg; ++g) {
+        const int64_t a = seg_k[g], b = seg_k[g + 1];
+        fseg_tile[g + 1] = fseg_tile[g] + (b > a ? (b - 1) / FLAG_CHUNK - a / FLAG_CHUNK + 1 : 0);
+    }
+    o.nftiles = fseg_tile[nseg];
+    GBK_TRY(upload(&o.fseg_tile, fseg_tile));
+    GBK_TRY(dev_alloc((void**)&o.ftiles, sizeof(int2) * (o.nftiles + 1)));
+    hipLaunchKernelGGL(k_flag_tiles, grid_e(o.nftiles + 1), dim3(256), 0, st, s.v.row_ptr, o.fseg_tile, o.seg_pair,
+                       (int)nseg, o.nftiles, nnz, o.ftiles);
+    GBK_LAUNCH_CHECK();
+    GBK_TRY(dev_alloc(&o.fcarry, 2 * vs * (size_t)(o.nftiles > 0 ? o.nftiles : 1)));
+    (void)seg_pair;
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}
+
+static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int64_t nseg, int64_t w, SegCSR& s) {
     hipStream_t st = stream();
     const size_t vs = type_size(type);
     const int64_t n = m.nrows, nnz = m.nnz;
@@ -789,8 +851,10 @@ static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, 
         GBK_TRY(corder.alloc(4 * (n > 0 ? n : 1)));
         GBK_TRY(ciota.alloc(4 * (n > 0 ? n : 1)));
         GBK_TRY(slot.alloc(4 * (n > 0 ? n : 1)));
-        GBK_HIP(hipMemsetAsync(deg.p, 0, 4 * (n + 1), st));
-        GBK_TRY(dev_count(m.col, nnz, deg.as<int32_t>(), st));
+        if (n > 0) {
+            hipLaunchKernelGGL(k_deg_from_ptr, grid_e(n), dim3(256), 0, st, optr, n, deg.as<int32_t>());
+            GBK_LAUNCH_CHECK();
+        }
         const int segbits = bits_for(nseg - 1);
         const int db = 32 - segbits;
         const size_t nh = (size_t)nseg * (H > 0 ? H : 1);
@@ -821,7 +885,7 @@ static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, 
         }
         GBK_HIP(hipStreamSynchronize(st));
     }
-    // wave tiles per segment
+    // first entry of every segment
     std::vector<int32_t> seg_k(nseg + 1);
     {
         TmpBuf sp_d, sk_d;
@@ -829,27 +893,26 @@ static int build_seg_device(const DevCSR& m, int type, int64_t nseg, int64_t w, 
         GBK_TRY(sk_d.alloc(4 * (nseg + 1)));
         GBK_HIP(hipMemcpyAsync(sp_d.p, seg_pair.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, st));
         GBK_TRY(dev_gather_i32(s.v.row_ptr, sp_d.as<int32_t>(), nseg + 1, sk_d.as<int32_t>(), st));
-        GBK_HIP(hipMemcpyAsync(seg_k.data(), sk_d.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, st));
-        GBK_HIP(hipStreamSynchronize(st));
+        GBK_TRY(d2h(seg_k.data(), sk_d.p, 4 * (nseg + 1), st));
     }
-    std::vector<int64_t> seg_tile(nseg + 1, 0);
-    for (int64_t g = 0; g < nseg; ++g) {
-        const int64_t items = (int64_t)(seg_pair[g + 1] - seg_pair[g]) + (int64_t)(seg_k[g + 1] - seg_k[g]);
-        seg_tile[g + 1] = seg_tile[g] + (items + HOT_WT - 1) / HOT_WT;
-    }
-    o.ntiles = seg_tile[nseg];
-    GBK_TRY(upload(&o.seg_tile, seg_tile));
     GBK_TRY(upload(&o.seg_pair, seg_pair));
-    GBK_TRY(dev_alloc((void**)&o.tiles, sizeof(int2) * (o.ntiles + 1)));
-    hipLaunchKernelGGL(k_seg_tiles, grid_e(o.ntiles + 1), dim3(256), 0, st, s.v.row_ptr, o.seg_tile, o.seg_pair,
-                       (int)nseg, o.ntiles, npairs, nnz, o.tiles);
-    GBK_LAUNCH_CHECK();
-    GBK_TRY(dev_alloc(&o.carry, 2 * vs * (size_t)(o.ntiles > 0 ? o.ntiles : 1)));
-    GBK_HIP(hipStreamSynchronize(st));
+    if (n <= (int64_t)0x40000000) {
+        // the flagged layout (columns below 2^30) replaces the column arrays
+        GBK_TRY(build_flagged(s, vs, seg_pair, seg_k));
+        dev_free(o.hcol);
+        o.hcol = nullptr;
+        dev_free(s.v.col);
+        s.v.col = nullptr;
+    } else {
+        dev_free(o.hcol);
+        o.hcol = nullptr;
+        GBK_TRY(seg_plain_tiles(s, type, seg_pair));
+    }
+    (void)npairs;
     return GBK_SUCCESS;
 }
 
-static int build_seg(const DevCSR& m, int type, SegCSR** out) {
+static int build_seg(const DevCSR& m, const DevCSR& other, int type, SegCSR** out) {
     const int64_t n = m.nrows;
     if (m.nnz >= (int64_t)INT32_MAX / 2) return fail(GBK_INVALID_VALUE, "too many entries for the segmented format");
     // a multiple of 8 segments (one group per XCD), each at most 512 Ki
@@ -858,7 +921,7 @@ static int build_seg(const DevCSR& m, int type, SegCSR** out) {
     const int64_t nseg = 8 * std::max<int64_t>(1, (n + 8 * sc - 1) / (8 * sc));
     const int64_t w = std::max<int64_t>(1, (n + nseg - 1) / nseg);
     SegCSR* s = new SegCSR();
-    const int rc = build_seg_device(m, type, nseg, w, *s);
+    const int rc = build_seg_device(m, other.row_ptr, type, nseg, w, *s);
     if (rc != GBK_SUCCESS) {
         free_seg_fields(*s);
         delete s;
@@ -921,31 +984,41 @@ __global__ void k_hot_x(const int32_t* __restrict__ ids, const T* __restrict__ x
     }
 }
 
-// Persistent merge-path SpMV over the (segment, row) pairs with the hot x of
-// the block's segment in LDS.  Grid: a multiple of 8 workgroups; workgroup b
-// runs on XCD b % 8 and serves segments b % 8, b % 8 + 8, ...; within a
-// segment, wave w of block j takes tiles seg_tile[s] + j * HOT_WAVES + w in
-// steps of (grid / 8) * HOT_WAVES.  Each wave merges its 512-item tile alone:
-// row i of the tile ends at merge item i + (its end offset), marked by a byte
-// in the wave's item-kind array (distinct bytes, no atomics); lane L reads
-// its 8 item kinds, its first row is a wave prefix sum of the per-lane
-// row-end counts (from four ballots), its item m is its
-// (m - row ends below m)-th nonzero, whose product is read from LDS before
-// the fold; a wave64 segmented scan joins rows split across lanes.  Sums of
-// rows completed in the tile go to partial[pair], the tile's trailing
-// partial and a deferred first row go to carry (k_hot_fix).
+// Eight consecutive elements from an address aligned to their size.
+typedef unsigned int gbk_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int gbk_u32x2 __attribute__((ext_vector_type(2)));
+template <typename V> __device__ __forceinline__ void load8_nt(const V* __restrict__ p, V (&out)[8]) {
+    if constexpr (sizeof(V) == 1) {
+        const gbk_u32x2 u = __builtin_nontemporal_load((const gbk_u32x2*)p);
+        __builtin_memcpy(out, &u, 8);
+    } else {
+        constexpr int NV = (int)(8 * sizeof(V) / 16);
+        gbk_u32x4 u[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) u[i] = __builtin_nontemporal_load((const gbk_u32x4*)p + i);
+        __builtin_memcpy(out, u, 8 * sizeof(V));
+    }
+}
+
+// Persistent SpMV over the flagged layout, hot x of the block's segment in
+// LDS.  Grid: a multiple of 8 workgroups; workgroup b runs on XCD b % 8 and
+// serves segments b % 8, b % 8 + 8, ...; within a segment, wave w of block j
+// takes tiles fseg_tile[s] + j * HOT_WAVES + w in steps of (grid / 8) *
+// HOT_WAVES.  Every (segment, row) pair has at least one entry, so a pair
+// ends exactly at an entry whose flag is set and tiles need no row items.  Lane L of a wave folds the
+// eight entries chunk + 8L .. 8L + 7 in order; its n-th flag ends pair
+// first_pair + (flags of lanes below L) + n, the prefix taken from four
+// ballots of the per-lane counts.  A lane's sum up to its first flag is
+// joined with the lanes below by a wave64 segmented scan; the tile's trailing
+// sum and a deferred first pair go to carry (k_flag_fix).
 template <typename T, int ADD, int MUL>
-__global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
-    const int32_t* __restrict__ vrp, const int32_t* __restrict__ hcol, const T* __restrict__ vval,
-    const T* __restrict__ x, const T* __restrict__ xh, int H, int nseg, T* __restrict__ partial,
-    const int2* __restrict__ tiles, const int64_t* __restrict__ seg_tile, T* __restrict__ carry) {
-    constexpr int WI = HOT_WT / 64;
-    static_assert(WI == 8, "eight item-kind bytes per lane");
+__global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
+    const int32_t* __restrict__ fcol, const T* __restrict__ vval, const T* __restrict__ x, const T* __restrict__ xh,
+    int H, int nseg, T* __restrict__ partial, const int2* __restrict__ ftiles, const int64_t* __restrict__ fseg_tile,
+    T* __restrict__ carry) {
     extern __shared__ __align__(16) unsigned char hot_lds[];
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T* s_x = (T*)hot_lds;
-    uint8_t* s_kind = hot_lds + (size_t)H * sizeof(T) + wid * HOT_WT;
-    T* s_prod = (T*)(hot_lds + (size_t)H * sizeof(T) + HOT_WAVES * HOT_WT) + wid * HOT_WT;
     const int j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
     const T ident = add_identity<ADD, T>();
     const uint64_t below = (1ull << lane) - 1ull;
@@ -953,131 +1026,126 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_hot(
         const T* src = xh + (int64_t)seg * H;
         for (int i = threadIdx.x; i < H; i += HOT_WAVES * 64) s_x[i] = src[i];
         __syncthreads();
-        const int64_t t_hi = seg_tile[seg + 1];
-        for (int64_t tile = seg_tile[seg] + (int64_t)j * HOT_WAVES + wid; tile < t_hi; tile += (int64_t)nbx * HOT_WAVES) {
-            const int2 c0 = tiles[tile], c1 = tiles[tile + 1];
-            const int r0 = c0.x, k0 = c0.y, nr = c1.x - r0, nk = c1.y - k0;
-            // row ends: unconditional loads of clamped (in-range) indices, so
-            // that all of them are in flight together
-            int rend[WI];
+        const int64_t t_lo = fseg_tile[seg], t_hi = fseg_tile[seg + 1];
+        const int64_t step = (int64_t)nbx * HOT_WAVES;
+        int64_t tile = t_lo + (int64_t)j * HOT_WAVES + wid;
+        if (tile < t_hi) {
+            // tile t of the segment covers (part of) chunk chunk0 + t - t_lo
+            const int chunk0 = ftiles[t_lo].x / FLAG_CHUNK;
+            int2 ti = ftiles[tile];
+            int k1 = ftiles[tile + 1].x;
+            int c[8];
+            T a[8];
+            {
+                const int base = (chunk0 + (int)(tile - t_lo)) * FLAG_CHUNK + lane * 8;
+                load8_nt(fcol + base, c);
+                load8_nt(vval + base, a);
+            }
+            while (true) {
+                // the next tile (clamped to the last one): its descriptor
+                // and, after this tile's gathers, its entries are loaded
+                // while this tile is reduced
+                const int64_t nt = min(tile + step, t_hi - 1);
+                const int2 ti_n = ftiles[nt];
+                const int k1_n = ftiles[nt + 1].x;
+                T xv[8];
 #pragma unroll
-            for (int m = 0; m < WI; ++m) rend[m] = vrp[(uint32_t)min(r0 + 1 + m * 64 + lane, r0 + nr)];
-            int ci[WI];
-            T av[WI];
-#pragma unroll
-            for (int it = 0; it < WI; ++it) {
-                const int q = it * 64 + lane;
-                ci[it] = 0;
-                av[it] = T(0);
-                if (q < nk) {
-                    ci[it] = __builtin_nontemporal_load(hcol + k0 + q);
-                    av[it] = __builtin_nontemporal_load(vval + k0 + q);
+                for (int m = 0; m < 8; ++m) {
+                    const int idx = c[m] & 0x3fffffff;
+                    if (c[m] & 0x40000000) xv[m] = s_x[idx];
+                    else xv[m] = x[(uint32_t)idx];
                 }
-            }
-            *(uint2*)(s_kind + lane * WI) = make_uint2(0u, 0u);
-            T xv[WI];
+                int cn[8];
+                T an[8];
+                {
+                    const int base_n = (chunk0 + (int)(nt - t_lo)) * FLAG_CHUNK + lane * 8;
+                    load8_nt(fcol + base_n, cn);
+                    load8_nt(vval + base_n, an);
+                }
+                const int k0 = ti.x;
+                const int pfirst = ti.y & 0x7fffffff;
+                const bool defer0 = ti.y < 0;
+                const int base = (k0 & ~(FLAG_CHUNK - 1)) + lane * 8;
+                // entries of this tile: [k0, k1) of the chunk
+                const int mlo = min(max(k0 - base, 0), 8), mhi = min(max(k1 - base, 0), 8);
+                const uint32_t valid = ((1u << mhi) - 1u) & ~((1u << mlo) - 1u);
+                uint32_t fl = 0;
 #pragma unroll
-            for (int it = 0; it < WI; ++it) {
-                const int q = it * 64 + lane;
-                xv[it] = T(0);
-                if (q < nk) xv[it] = ci[it] < 0 ? s_x[-1 - ci[it]] : x[(uint32_t)ci[it]];
-            }
-            // LDS operations of one wave complete in issue order: the kinds
-            // are cleared before they are marked, and written before they are read
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+                for (int m = 0; m < 8; ++m) fl |= ((uint32_t)c[m] >> 31) << m;
+                fl &= valid;
+                const int cnt = __builtin_popcount(fl);
+                int before = 0;
 #pragma unroll
-            for (int m = 0; m < WI; ++m) {
-                const int i = m * 64 + lane;
-                if (i < nr) s_kind[i + rend[m] - k0] = 1;
-            }
+                for (int q = 0; q < 4; ++q) before += __builtin_popcountll(__ballot((cnt >> q) & 1) & below) << q;
+                T acc = ident, firstsum = ident;
+                int nf = 0;
+                const int pbase = pfirst + before;
 #pragma unroll
-            for (int it = 0; it < WI; ++it) {
-                const int q = it * 64 + lane;
-                if (q < nk) s_prod[q] = mul_op<MUL, T>(av[it], xv[it]);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            const int total = nr + nk;
-            const uint2 kw = *(const uint2*)(s_kind + lane * WI);
-            const uint64_t kb = ((uint64_t)kw.y << 32) | kw.x;
-            const int nitems = min(max(total - lane * WI, 0), WI);
-            uint32_t bits = 0;
-#pragma unroll
-            for (int m = 0; m < WI; ++m) bits |= (uint32_t)((kb >> (8 * m)) & 1ull) << m;
-            bits &= (1u << nitems) - 1u;
-            const int cnt = __builtin_popcount(bits);
-            int before = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) before += __builtin_popcountll(__ballot((cnt >> q) & 1) & below) << q;
-            const int kj = lane * WI - before;
-            T pv[WI];
-#pragma unroll
-            for (int m = 0; m < WI; ++m) pv[m] = s_prod[min(kj + m - __builtin_popcount(bits & ((1u << m) - 1u)), HOT_WT - 1)];
-            T acc = ident, firstsum = ident;
-            int first = -1, ri = before;
-#pragma unroll
-            for (int m = 0; m < WI; ++m) {
-                const bool isend = (bits >> m) & 1u;
-                if (isend) {
-                    if (first < 0) {
-                        first = ri;
-                        firstsum = acc;
-                    } else {
-                        partial[r0 + ri] = acc;
+                for (int m = 0; m < 8; ++m) {
+                    const T pm = ((valid >> m) & 1u) ? mul_op<MUL, T>(a[m], xv[m]) : ident;
+                    acc = add_op<ADD, T>(acc, pm);
+                    if ((fl >> m) & 1u) {
+                        if (nf == 0) firstsum = acc;
+                        else partial[pbase + nf] = acc;
+                        ++nf;
+                        acc = ident;
                     }
-                    ++ri;
                 }
-                acc = isend ? ident : (m < nitems ? add_op<ADD, T>(acc, pv[m]) : acc);
-            }
-            // segmented scan of (trailing partial, has-row-end) over the wave
-            T v = acc;
-            int f = first >= 0;
+                // segmented scan of (trailing sum, has-flag) over the wave
+                T v = acc;
+                int f = cnt > 0;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const T ov = hot_shfl_up(v, off);
-                const int of = __shfl_up(f, off, 64);
-                if (lane >= off) {
-                    if (!f) v = add_op<ADD, T>(ov, v);
-                    f |= of;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const T ov = hot_shfl_up(v, off);
+                    const int of = __shfl_up(f, off, 64);
+                    if (lane >= off) {
+                        if (!f) v = add_op<ADD, T>(ov, v);
+                        f |= of;
+                    }
+                }
+                T ev = hot_shfl_up(v, 1);
+                if (lane == 0) ev = ident;
+                const T tail = hot_shfl(v, 63);
+                if (cnt > 0) {
+                    const T tot = add_op<ADD, T>(ev, firstsum);
+                    if (before == 0 && defer0) carry[2 * tile + 1] = tot;
+                    else partial[pbase] = tot;
+                }
+                if (lane == 0) carry[2 * tile] = tail;
+                tile += step;
+                if (tile >= t_hi) break;
+                ti = ti_n;
+                k1 = k1_n;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    c[m] = cn[m];
+                    a[m] = an[m];
                 }
             }
-            T ev = hot_shfl_up(v, 1);
-            if (lane == 0) ev = ident;
-            const T tail = hot_shfl(v, 63);
-            const bool defer0 = (nr > 0) && (vrp[r0] < k0);
-            if (first >= 0) {
-                const T tot = add_op<ADD, T>(ev, firstsum);
-                if (first == 0 && defer0) carry[2 * tile + 1] = tot;
-                else partial[r0 + first] = tot;
-            }
-            if (lane == 0) carry[2 * tile] = tail;
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
     }
 }
 
-// Rows that began in an earlier tile: their sum is the deferred part plus the
-// trailing partials of the tiles back to the one holding the row start, in
-// tile order.
+// Pairs that began in an earlier tile and end in tile t: the deferred part
+// plus the trailing sums of the tiles back to the one holding the pair start,
+// in tile order.
 template <typename T, int ADD>
-__global__ void k_hot_fix(const int32_t* __restrict__ vrp, const int2* __restrict__ tiles, int64_t ntiles,
-                          const T* __restrict__ carry, T* __restrict__ partial) {
+__global__ void k_flag_fix(const int32_t* __restrict__ vrp, const int2* __restrict__ ftiles, int64_t ntiles,
+                           const T* __restrict__ carry, T* __restrict__ partial) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
-    const int2 c0 = tiles[t], c1 = tiles[t + 1];
-    const int r0 = c0.x, k0 = c0.y;
-    if (c1.x == r0) return;
-    const int start = vrp[r0];
-    if (start >= k0) return;
+    const int2 ti = ftiles[t];
+    if (ti.y >= 0) return;
+    const int p = ti.y & 0x7fffffff;
+    if (vrp[p + 1] > ftiles[t + 1].x) return;  // the pair does not end in this tile
+    const int start = vrp[p];
     T total = carry[2 * t + 1];
     for (int64_t u = t - 1; u >= 0; --u) {
         total = add_op<ADD, T>(carry[2 * u], total);
-        if (tiles[u].y <= start) break;
+        if (ftiles[u].x <= start) break;
     }
-    partial[r0] = total;
+    partial[p] = total;
 }
 
 template <typename T, int ADD, int MUL> static int hot_prepare() {
@@ -1085,7 +1153,7 @@ template <typename T, int ADD, int MUL> static int hot_prepare() {
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-        err = hipFuncSetAttribute((const void*)k_seg_hot<T, ADD, MUL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        err = hipFuncSetAttribute((const void*)k_seg_flag<T, ADD, MUL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)device_lds_bytes());
     });
     if (err != hipSuccess) return fail(GBK_PANIC, "hipFuncSetAttribute: %s", hipGetErrorString(err));
@@ -1102,16 +1170,16 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
                            (T*)h.xh, nh);
         GBK_LAUNCH_CHECK();
     }
-    if (h.ntiles > 0) {
+    if (h.nftiles > 0) {
         GBK_TRY((hot_prepare<T, ADD, MUL>()));
-        const size_t lds = (size_t)h.H * sizeof(T) + hot_staging_bytes(sizeof(T));
+        const size_t lds = (size_t)h.H * sizeof(T);
         const int grid = 8 * std::max(1, device_cu_count() / 8);
-        hipLaunchKernelGGL((k_seg_hot<T, ADD, MUL>), dim3(grid), dim3(HOT_WAVES * 64), lds, st, s->v.row_ptr, h.hcol,
-                           (const T*)s->v.val, x, (const T*)h.xh, h.H, s->nseg, (T*)s->partial, h.tiles,
-                           h.seg_tile, (T*)h.carry);
+        hipLaunchKernelGGL((k_seg_flag<T, ADD, MUL>), dim3(grid), dim3(HOT_WAVES * 64), lds, st, h.fcol,
+                           (const T*)s->v.val, x, (const T*)h.xh, h.H, s->nseg, (T*)s->partial, h.ftiles,
+                           h.fseg_tile, (T*)h.fcarry);
         GBK_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_hot_fix<T, ADD>), dim3((unsigned)((h.ntiles + 255) / 256)), dim3(256), 0, st,
-                           s->v.row_ptr, h.tiles, h.ntiles, (const T*)h.carry, (T*)s->partial);
+        hipLaunchKernelGGL((k_flag_fix<T, ADD>), dim3((unsigned)((h.nftiles + 255) / 256)), dim3(256), 0, st,
+                           s->v.row_ptr, h.ftiles, h.nftiles, (const T*)h.fcarry, (T*)s->partial);
         GBK_LAUNCH_CHECK();
     }
     if (s->n > 0) {
@@ -1127,7 +1195,7 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
 }
 
 static int seg_spmv(const SegCSR* s, int type, const void* x, void* y, int add, int mul, int accum) {
-    if (s->hot.tiles && !order_sensitive_monoid(add)) {
+    if (s->hot.fcol && !order_sensitive_monoid(add)) {
         return dispatch_type(type, [&](auto tv) -> int {
             typedef decltype(tv) T;
             return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
@@ -1188,7 +1256,7 @@ __global__ void k_lav_key(const int32_t* __restrict__ col, const int32_t* __rest
     key[e] = (uint32_t)(c < K ? c / w : ndense);
 }
 
-static int build_lav_device(const DevCSR& m, int type, double dense_frac, int64_t w, Lav& l) {
+static int build_lav_device(const DevCSR& m, const int32_t* optr, int type, double dense_frac, int64_t w, Lav& l) {
     hipStream_t st = stream();
     const int64_t n = m.nrows, nnz = m.nnz;
     const int64_t n1 = n > 0 ? n : 1, z1 = nnz > 0 ? nnz : 1;
@@ -1203,11 +1271,11 @@ static int build_lav_device(const DevCSR& m, int type, double dense_frac, int64_
     GBK_TRY(sdeg.alloc(4 * n1));
     GBK_TRY(excl.alloc(4 * (n1 + 1)));
     GBK_TRY(Kd.alloc(4));
-    GBK_HIP(hipMemsetAsync(deg.p, 0, 4 * (n1 + 1), st));
     GBK_HIP(hipMemsetAsync(Kd.p, 0, 4, st));
-    GBK_TRY(dev_count(m.col, nnz, deg.as<int32_t>(), st));
     int32_t K = 0;
     if (n > 0) {
+        hipLaunchKernelGGL(k_deg_from_ptr, grid_b(n), dim3(256), 0, st, optr, n, deg.as<int32_t>());
+        GBK_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_desc_key, grid_b(n), dim3(256), 0, st, deg.as<int32_t>(), n, key.as<uint32_t>());
         GBK_LAUNCH_CHECK();
         GBK_TRY(dev_iota(ids.as<int32_t>(), n, st));
@@ -1237,19 +1305,12 @@ static int build_lav_device(const DevCSR& m, int type, double dense_frac, int64_
         }
         GBK_TRY(seg_pairs_device(m, type, nseg, ekey.as<uint32_t>(), o2n.as<int32_t>(), l.seg, seg_pair));
     }
-    // merge-path tiles of the virtual CSR, ordered so a segment's tiles run on one XCD
-    SegCSR& s = l.seg;
-    GBK_TRY(build_merge_tiles(s.v, type));
-    std::vector<int2> tiles(s.v.ntiles + 1);
-    GBK_TRY(d2h(tiles.data(), s.v.tiles, sizeof(int2) * (s.v.ntiles + 1), st));
-    std::vector<int64_t> sp(seg_pair.begin(), seg_pair.end());
-    GBK_TRY(upload(&s.tile_map, xcd_tile_map(tiles, sp, (int)nseg)));
-    return GBK_SUCCESS;
+    return seg_plain_tiles(l.seg, type, seg_pair);
 }
 
-static int build_lav(const DevCSR& m, int type, Lav** out) {
+static int build_lav(const DevCSR& m, const DevCSR& other, int type, Lav** out) {
     Lav* l = new Lav();
-    const int rc = build_lav_device(m, type, 0.8, seg_cols_default(m.nrows), *l);
+    const int rc = build_lav_device(m, other.row_ptr, type, 0.8, seg_cols_default(m.nrows), *l);
     if (rc != GBK_SUCCESS) {
         free_seg_fields(l->seg);
         dev_free(l->new_to_old);
@@ -1306,10 +1367,11 @@ int format_build(gbk_matrix A) {
     format_free(A);
     for (int t = 0; t < 2; ++t) {
         const DevCSR& m = t ? A->csc : A->csr;
+        const DevCSR& other = t ? A->csr : A->csc;  // its row lengths are m's column counts
         int rc = GBK_SUCCESS;
         if (A->format == GBK_FMT_SELL_C) rc = build_sell(m, A->type, A->sell_c, A->sigma, &A->sell[t]);
-        else if (A->format == GBK_FMT_CSR_SEG) rc = build_seg(m, A->type, &A->seg[t]);
-        else if (A->format == GBK_FMT_LAV) rc = build_lav(m, A->type, &A->lav[t]);
+        else if (A->format == GBK_FMT_CSR_SEG) rc = build_seg(m, other, A->type, &A->seg[t]);
+        else if (A->format == GBK_FMT_LAV) rc = build_lav(m, other, A->type, &A->lav[t]);
         if (rc != GBK_SUCCESS) {
             format_free(A);
             return rc;
@@ -1351,10 +1413,11 @@ int64_t format_bytes(gbk_matrix A) {
             b += (int64_t)(s->padded * (4 + vs) + s->nchunks * (8 + 4 + 8 * s->C) + (s->nproblem + s->nlong) * 4 +
                            s->lng.nnz * (4 + vs) + s->lng.nrows * 4);
         const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
-        if (sg) b += (int64_t)(sg->npairs * (4 + 4 + vs + 4) + sg->v.nnz * (4 + vs) + sg->n * 4 + sg->v.ntiles * 28);
-        if (sg && sg->hot.tiles)
-            b += (int64_t)(sg->v.nnz * 4 + (int64_t)sg->nseg * sg->hot.H * (4 + vs) + sg->hot.ntiles * (8 + 2 * vs) +
-                           (sg->nseg + 1) * 8);
+        if (sg) b += (int64_t)(sg->npairs * (4 + 4 + vs + 4) + sg->v.nnz * vs + sg->n * 4 + sg->v.ntiles * 28);
+        if (sg && sg->v.col) b += sg->v.nnz * 4;
+        if (sg && sg->hot.fcol)
+            b += (int64_t)(sg->v.nnz * 4 + (int64_t)sg->nseg * sg->hot.H * (4 + vs) + sg->hot.nftiles * (8 + 2 * vs) +
+                           (sg->nseg + 1) * 16);
         if (A->lav[t]) b += A->lav[t]->seg.n * (4 + vs);
     }
     return b;

@@ -163,7 +163,7 @@ int dev_sort_pairs64(const uint64_t* kin, uint64_t* kout, const int32_t* vin, in
 int dev_exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t m, hipStream_t st);
 int dev_iota(int32_t* v, int64_t n, hipStream_t st);
 int dev_row_ids(const int32_t* rp, int64_t n, int32_t* rowid, hipStream_t st);
-int dev_count(const int32_t* key, int64_t m, int32_t* cnt, hipStream_t st);
+int dev_bounds_sorted(const uint32_t* key, int64_t m, int64_t nbins, int32_t* ptr, hipStream_t st);
 int dev_gather_i32(const int32_t* src, const int32_t* idx, int64_t m, int32_t* out, hipStream_t st);
 int dev_gather_bytes(const void* src, const int32_t* idx, int64_t m, size_t vs, void* out, hipStream_t st);
 int device_transpose(const DevCSR& in, int type, int64_t n, DevCSR& out);
