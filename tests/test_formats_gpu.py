@@ -107,9 +107,10 @@ def _seg_case(gbk, oracle, rp, col, vals, x, sr, exact=True):
 def test_seg_hot_columns_every_type(gbk, oracle):
     """The segmented format's hot-column kernel (x of a segment's highest
     in-degree columns in LDS) for every value type; order-sensitive monoids
-    take the sequential path.  R-MAT 18 has more columns per segment than LDS
-    slots, so hot and L2 columns mix in every tile."""
-    graphs = {"rmat18": gbk.gen_rmat(18, 8, seed=61), "rmat12": gbk.gen_rmat(12, 16, seed=62)}
+    take the sequential path.  R-MAT 19 has more columns per segment (65,536)
+    than LDS slots for 4- and 8-byte values (40,960 and 20,480), so hot and
+    L2 columns mix in every tile."""
+    graphs = {"rmat19": gbk.gen_rmat(19, 8, seed=61), "rmat12": gbk.gen_rmat(12, 16, seed=62)}
     graphs.update(special_graphs())
     for gname, (rp, col) in graphs.items():
         nnz = len(col)
