@@ -500,7 +500,40 @@ def tc_bench(g, scale):
     res["cpu_sandia"] = {"triangles": cpu, "seconds": round(tc, 3), "cores": 1, "kind": "port"}
     log(f"oracle (1 core) Sandia: {cpu} triangles in {tc:.2f} s")
     A.free()
+    res["mxm_unmasked"] = spgemm_bench(g, O)
     return res
+
+
+def spgemm_bench(g, O, scale=14):
+    """Unmasked C = A (+.*) A (int32 plus.times) on a symmetric R-MAT: device
+    time, products expanded and entries of C, beside the oracle's sequential
+    IKJ product on one host core."""
+    rp, col = g.gen_rmat(scale, 16, seed=48, symmetric=True)
+    n = len(rp) - 1
+    v = (np.arange(len(col)) % 7 + 1).astype(np.int32)
+    A = g.Matrix.from_csr(rp, col, v)
+    Cm = g.Matrix(g.INT32)
+    g.mxm(Cm, None, g.ACC_NONE, "PlusMultiplies", A, A)
+    times = []
+    for _ in range(3):
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.mxm(Cm, None, g.ACC_NONE, "PlusMultiplies", A, A)
+        times.append(time.perf_counter() - t0)
+    products = int(np.diff(rp)[col].sum())
+    cnnz = Cm.nnz
+    t0 = time.perf_counter()
+    _, ccol, _ = O.mxm_unmasked(rp, col, v, rp, col, v, 0, 0)
+    tc = time.perf_counter() - t0
+    ms = float(np.median(times)) * 1e3
+    log(f"mxm unmasked symmetric R-MAT {scale}: {products} products -> {cnnz} entries in {ms:.1f} ms "
+        f"({products / (ms * 1e-3) / 1e9:.2f} G products/s); oracle (1 core) {tc:.2f} s")
+    assert len(ccol) == cnnz
+    A.free()
+    Cm.free()
+    return {"graph": f"symmetric R-MAT scale {scale}, edge factor 16", "n": n, "stored_nnz": len(col),
+            "products": products, "c_nnz": cnnz, "ms_median": round(ms, 2),
+            "cpu_ikj": {"seconds": round(tc, 3), "cores": 1, "kind": "port"}}
 
 
 def io_bench(g, scale, ef):

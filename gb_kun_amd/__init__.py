@@ -458,7 +458,8 @@ def vxm(w, mask, accum, semiring, u, A, desc_mask=DEFAULT):
 
 
 def mxm(Cm, mask, accum, semiring, A, B, desc_mask=DEFAULT):
-    """C<mask> = A (+.*) B (structural mask; C gets the mask's entries)."""
+    """C<mask> = A (+.*) B (structural mask: C gets the mask's entries; mask None:
+    the unmasked product, entries whose fold is 0 dropped)."""
     add, mul = SEMIRINGS[semiring] if isinstance(semiring, str) else semiring
     check(lib.gbk_mxm(Cm.h, mask.h if mask is not None else None, accum, Semiring(add, mul), A.h, B.h,
                       _desc(desc_mask)))

@@ -1,4 +1,4 @@
-// Masked sparse matrix-matrix product and triangle counting.
+// Sparse matrix-matrix products (masked and unmasked) and triangle counting.
 //
 // Reference: lablas::mxm (src/cpp_graphblas/operations.hpp:411-437),
 // backend::mxm (src/backend/operations/operations.h:436-486) with a mask:
@@ -302,6 +302,200 @@ int masked_count(const DevCSR& M, const DevCSR& A, const DevCSR& B, uint64_t* ou
     return GBK_SUCCESS;
 }
 
+// ------------------------------------------------------- unmasked product --
+// C = A (+.*) B without a mask (SpMSpM_unmasked_ikj,
+// src/backend/spmspm/spmspm_hash_based_gustavson.h:9-153): expand, sort,
+// compress.  Every product A(i,k) B(k,j) is written at a position in IKJ
+// order (A's entries in row order, then row k of B in order) with the key
+// i n + j; a stable radix sort of the keys keeps that order within a key, so
+// one thread folding a key's products from the identity forms the
+// reference's C(i,j) = add(... add(identity, p1) ..., pm) bit for bit;
+// entries whose value is 0 are dropped, as the reference does.  Columns come
+// out ascending (the reference writes them in hash-map order).
+
+__global__ void k_prod_count(const int32_t* __restrict__ acol, int64_t annz, const int32_t* __restrict__ brp,
+                             int64_t* __restrict__ cnt) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < annz) cnt[e] = brp[acol[e] + 1] - brp[acol[e]];
+}
+
+// A group of MXM_G lanes per entry of A writes its row of B's products.
+template <typename T, int MUL>
+__global__ __launch_bounds__(256) void k_expand(const int32_t* __restrict__ arow, const int32_t* __restrict__ acol,
+                                                const T* __restrict__ aval, int64_t annz,
+                                                const int32_t* __restrict__ brp, const int32_t* __restrict__ bcol,
+                                                const T* __restrict__ bval, const int64_t* __restrict__ offs,
+                                                int64_t n, uint64_t* __restrict__ keys, T* __restrict__ prod) {
+    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / MXM_G;
+    const int q = threadIdx.x % MXM_G;
+    if (g >= annz) return;
+    const uint64_t rowkey = (uint64_t)arow[g] * (uint64_t)n;
+    const int k = acol[g];
+    const T a = aval[g];
+    const int b0 = brp[k], len = brp[k + 1] - b0;
+    const int64_t o = offs[g];
+    for (int t = q; t < len; t += MXM_G) {
+        keys[o + t] = rowkey + (uint64_t)bcol[b0 + t];
+        prod[o + t] = mul_op<MUL, T>(a, bval[b0 + t]);
+    }
+}
+
+__global__ void k_key_starts(const uint64_t* __restrict__ skeys, int64_t F, int32_t* __restrict__ flag) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < F) flag[p] = (p == 0 || skeys[p] != skeys[p - 1]) ? 1 : 0;
+}
+
+__global__ void k_run_starts(const int32_t* __restrict__ flag, const int32_t* __restrict__ uidx, int64_t F,
+                             int64_t U, int32_t* __restrict__ start) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < F && flag[p]) start[uidx[p]] = (int32_t)p;
+    if (p == 0) start[U] = (int32_t)F;
+}
+
+// One thread per distinct key: the left fold of its products in sorted
+// (= IKJ) order; keep[u] = (sum != 0).
+template <typename T, int ADD>
+__global__ void k_run_fold(const uint64_t* __restrict__ skeys, const int32_t* __restrict__ perm,
+                           const T* __restrict__ prod, const int32_t* __restrict__ start, int64_t U, int64_t n,
+                           int32_t* __restrict__ urow, int32_t* __restrict__ ucol, T* __restrict__ uval,
+                           int32_t* __restrict__ keep) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= U) return;
+    const int32_t p0 = start[u], p1 = start[u + 1];
+    T acc = add_identity<ADD, T>();
+    for (int32_t p = p0; p < p1; ++p) acc = add_op<ADD, T>(acc, prod[perm[p]]);
+    const uint64_t key = skeys[p0];
+    urow[u] = (int32_t)(key / (uint64_t)n);
+    ucol[u] = (int32_t)(key % (uint64_t)n);
+    uval[u] = acc;
+    keep[u] = acc != T(0) ? 1 : 0;
+}
+
+template <typename T>
+__global__ void k_keep_compact(const int32_t* __restrict__ keep, const int32_t* __restrict__ kidx, int64_t U,
+                               const int32_t* __restrict__ urow, const int32_t* __restrict__ ucol,
+                               const T* __restrict__ uval, int32_t* __restrict__ crow, int32_t* __restrict__ ccol,
+                               T* __restrict__ cval) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= U || !keep[u]) return;
+    const int32_t o = kidx[u];
+    crow[o] = urow[u];
+    ccol[o] = ucol[u];
+    cval[o] = uval[u];
+}
+
+// C's CSR arrays (allocated here, handed to the caller) of A (+.*) B.
+int unmasked_product(const DevCSR& A, const DevCSR& B, int type, int add, int mul, int64_t n, int32_t** crp_out,
+                     int32_t** ccol_out, void** cval_out, int64_t* nnz_out) {
+    hipStream_t st = stream();
+    const size_t vs = type_size(type);
+    const int64_t annz = A.nnz;
+    DevBuf cnt, offs, arow;
+    GBK_TRY(cnt.alloc(8 * (annz + 1)));
+    GBK_TRY(offs.alloc(8 * (annz + 1)));
+    GBK_TRY(arow.alloc(4 * (annz > 0 ? annz : 1)));
+    GBK_HIP(hipMemsetAsync(cnt.as<int64_t>() + annz, 0, 8, st));
+    if (annz > 0) {
+        hipLaunchKernelGGL(k_prod_count, grid_m(annz), dim3(256), 0, st, A.col, annz, B.row_ptr, cnt.as<int64_t>());
+        GBK_LAUNCH_CHECK();
+    }
+    GBK_TRY(dev_exclusive_scan_i64(cnt.as<int64_t>(), offs.as<int64_t>(), annz + 1, st));
+    int64_t F = 0;
+    GBK_HIP(hipMemcpyAsync(&F, offs.as<int64_t>() + annz, 8, hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    if (F >= (int64_t)INT32_MAX) return fail(GBK_INVALID_VALUE, "mxm: %lld products exceed 2^31", (long long)F);
+    if (n > 0 && (uint64_t)n > (~0ull) / (uint64_t)n) return fail(GBK_INVALID_VALUE, "mxm: dimension too large");
+    GBK_TRY(dev_row_ids(A.row_ptr, n, arow.as<int32_t>(), st));
+    const int64_t F1 = F > 0 ? F : 1;
+    DevBuf keys, skeys, prod, iota, perm, flag, uidx;
+    GBK_TRY(keys.alloc(8 * F1));
+    GBK_TRY(skeys.alloc(8 * F1));
+    GBK_TRY(prod.alloc(vs * F1));
+    GBK_TRY(iota.alloc(4 * F1));
+    GBK_TRY(perm.alloc(4 * F1));
+    GBK_TRY(flag.alloc(4 * F1));
+    GBK_TRY(uidx.alloc(4 * (F1 + 1)));
+    int rc = dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int MUL = decltype(b)::value;
+            (void)a;
+            if (annz > 0 && F > 0) {
+                hipLaunchKernelGGL((k_expand<T, MUL>), grid_m(annz * MXM_G), dim3(256), 0, st, arow.as<int32_t>(),
+                                   A.col, (const T*)A.val, annz, B.row_ptr, B.col, (const T*)B.val,
+                                   offs.as<int64_t>(), n, keys.as<uint64_t>(), prod.as<T>());
+                GBK_LAUNCH_CHECK();
+            }
+            return GBK_SUCCESS;
+        });
+    });
+    GBK_TRY(rc);
+    GBK_TRY(dev_iota(iota.as<int32_t>(), F, st));
+    const uint64_t maxkey = n > 0 ? (uint64_t)n * (uint64_t)n - 1 : 0;
+    int kb = 1;
+    while (kb < 64 && (maxkey >> kb) != 0) ++kb;
+    GBK_TRY(dev_sort_pairs64(keys.as<uint64_t>(), skeys.as<uint64_t>(), iota.as<int32_t>(), perm.as<int32_t>(), F,
+                             kb, st));
+    int32_t U = 0;
+    if (F > 0) {
+        hipLaunchKernelGGL(k_key_starts, grid_m(F), dim3(256), 0, st, skeys.as<uint64_t>(), F, flag.as<int32_t>());
+        GBK_LAUNCH_CHECK();
+        GBK_TRY(exclusive_scan_i32(flag.as<int32_t>(), uidx.as<int32_t>(), F, &U));
+    }
+    const int64_t U1 = U > 0 ? U : 1;
+    DevBuf start, urow, ucol, uval, keep, kidx, crow;
+    GBK_TRY(start.alloc(4 * (U1 + 1)));
+    GBK_TRY(urow.alloc(4 * U1));
+    GBK_TRY(ucol.alloc(4 * U1));
+    GBK_TRY(uval.alloc(vs * U1));
+    GBK_TRY(keep.alloc(4 * U1));
+    GBK_TRY(kidx.alloc(4 * (U1 + 1)));
+    int32_t nz = 0;
+    int32_t *crp = nullptr, *ccol = nullptr;
+    void* cval = nullptr;
+    rc = dispatch_type(type, [&](auto tv) -> int {
+        typedef decltype(tv) T;
+        return dispatch_semiring(add, mul, [&](auto a, auto b) -> int {
+            constexpr int ADD = decltype(a)::value;
+            (void)b;
+            if (U > 0) {
+                hipLaunchKernelGGL(k_run_starts, grid_m(F), dim3(256), 0, st, flag.as<int32_t>(), uidx.as<int32_t>(),
+                                   F, (int64_t)U, start.as<int32_t>());
+                GBK_LAUNCH_CHECK();
+                hipLaunchKernelGGL((k_run_fold<T, ADD>), grid_m(U), dim3(256), 0, st, skeys.as<uint64_t>(),
+                                   perm.as<int32_t>(), prod.as<T>(), start.as<int32_t>(), (int64_t)U, n,
+                                   urow.as<int32_t>(), ucol.as<int32_t>(), uval.as<T>(), keep.as<int32_t>());
+                GBK_LAUNCH_CHECK();
+                GBK_TRY(exclusive_scan_i32(keep.as<int32_t>(), kidx.as<int32_t>(), U, &nz));
+            }
+            const int64_t nz1 = nz > 0 ? nz : 1;
+            GBK_TRY(crow.alloc(4 * nz1));
+            GBK_TRY(dev_alloc((void**)&crp, 4 * (n + 1)));
+            GBK_TRY(dev_alloc((void**)&ccol, 4 * nz1));
+            GBK_TRY(dev_alloc(&cval, vs * nz1));
+            if (nz > 0) {
+                hipLaunchKernelGGL(k_keep_compact<T>, grid_m(U), dim3(256), 0, st, keep.as<int32_t>(),
+                                   kidx.as<int32_t>(), (int64_t)U, urow.as<int32_t>(), ucol.as<int32_t>(),
+                                   uval.as<T>(), crow.as<int32_t>(), ccol, (T*)cval);
+                GBK_LAUNCH_CHECK();
+            }
+            return dev_bounds_sorted((const uint32_t*)crow.p, nz, n, crp, st);
+        });
+    });
+    if (rc == GBK_SUCCESS && hipStreamSynchronize(st) != hipSuccess) rc = fail(GBK_PANIC, "sync failed");
+    if (rc != GBK_SUCCESS) {
+        dev_free(crp);
+        dev_free(ccol);
+        dev_free(cval);
+        return rc;
+    }
+    *crp_out = crp;
+    *ccol_out = ccol;
+    *cval_out = cval;
+    *nnz_out = nz;
+    return GBK_SUCCESS;
+}
+
 }  // namespace
 
 }  // namespace gbk
@@ -322,7 +516,17 @@ extern "C" int gbk_mxm(gbk_matrix C, gbk_matrix M, int accum, gbk_semiring op, g
     GBK_TRY(check_built(B));
     if (desc && (desc->mask == GBK_COMP || desc->mask == GBK_STR_COMP))
         return fail(GBK_INVALID_VALUE, "Error: complementary mask is not supported yet");
-    if (!M) return fail(GBK_INVALID_VALUE, "mxm without a mask is not supported");
+    if (!M) {
+        if (A->n != B->n) return fail(GBK_DIMENSION_MISMATCH, "mxm operands differ in size");
+        if (A->type != B->type) return fail(GBK_DOMAIN_MISMATCH, "mxm operands differ in type");
+        if (C == A || C == B) return fail(GBK_INVALID_VALUE, "mxm result aliases an operand");
+        int32_t *crp = nullptr, *ccol = nullptr;
+        void* cval = nullptr;
+        int64_t nz = 0;
+        GBK_TRY(unmasked_product(A->csr, B->csr, A->type, op.add, op.mul, A->n, &crp, &ccol, &cval, &nz));
+        C->type = A->type;
+        return build_from_device_csr(C, A->n, nz, crp, ccol, cval);
+    }
     GBK_TRY(check_built(M));
     if (A->n != B->n || A->n != M->n) return fail(GBK_DIMENSION_MISMATCH, "mxm operands differ in size");
     if (A->type != B->type) return fail(GBK_DOMAIN_MISMATCH, "mxm operands differ in type");

@@ -351,13 +351,15 @@ int gbk_sssp(gbk_matrix A, int64_t source, void* dist, int64_t* stats);
 /* Same, writing the distances (n entries of the matrix type) to a device array. */
 int gbk_sssp_device(gbk_matrix A, int64_t source, void* dist_dev, int64_t* stats);
 
-/* C<M> = A (+.*) B with a structural mask, replaces lablas::mxm with a mask
- * (src/cpp_graphblas/operations.hpp:411-437; backend::mxm, operations.h:
- * 436-486, masked IKJ: spmspm_masked_hash_based_gustavson.h:9-170).  C gets
- * exactly the entries of M; C(i,j) = fold over k of A(i,k) (*) B(k,j), 0 where
- * no k contributes.  A, B, M are n x n; A and B of one type, which C takes.
- * accum is ignored (the reference rebuilds C); a complemented mask or a NULL
- * mask is an error. */
+/* C<M> = A (+.*) B, replaces lablas::mxm (src/cpp_graphblas/operations.hpp:
+ * 411-437; backend::mxm, operations.h:436-486).  With a structural mask M
+ * (masked IKJ: spmspm_masked_hash_based_gustavson.h:9-170) C gets exactly the
+ * entries of M; C(i,j) = fold over k of A(i,k) (*) B(k,j), 0 where no k
+ * contributes.  With M == NULL (unmasked IKJ: spmspm_hash_based_gustavson.h:
+ * 9-153) C holds every (i,j) some k reaches whose fold is not 0, the fold in
+ * IKJ order from the identity, columns ascending.  A, B (and M) are n x n; A
+ * and B of one type, which C takes.  accum is ignored (the reference rebuilds
+ * C); a complemented mask is an error. */
 int gbk_mxm(gbk_matrix C, gbk_matrix M, int accum, gbk_semiring op, gbk_matrix A, gbk_matrix B,
             const gbk_desc* desc);
 

@@ -372,7 +372,8 @@ LA_Info vxm(Vector<W>* w, const Vector<M>* mask, NULL_TYPE, SemiringT op, const 
     return vxm(w, mask, second<U, W, U>(), op, u, A, desc);
 }
 
-// mxm with a mask (operations.hpp:411-437): C gets the mask's entries.
+// mxm (operations.hpp:411-437): with a mask C gets the mask's entries; a NULL
+// mask gives the unmasked product (entries whose fold is 0 dropped).
 template <typename c, typename m, typename a, typename b, typename BinaryOpTAccum, typename SemiringT>
 LA_Info mxm(Matrix<c>* C, const Matrix<m>* mask, BinaryOpTAccum, SemiringT, const Matrix<a>* A, const Matrix<b>* B,
             Descriptor* desc) {

@@ -23,6 +23,8 @@
  *   orc_pagerank_f64    golden PageRank of the compiled reference; fixed point
  *   orc_ppr_f64         closed-form linear solve; uniform teleport reduces it
  *                       to orc_pagerank_f64
+ *   orc_mxm_unmasked    scipy sparse products (int64 plus.times), A * I = A,
+ *                       brute-force min.plus on small matrices
  *   orc_mxm_masked      dense products on small matrices (plus.times,
  *                       min.plus), restricted to the mask's pattern
  *   orc_masked_count    trace(A^3) / 6 of symmetric graphs (scipy)
@@ -659,6 +661,78 @@ int orc_mxm_masked(int type, int64_t n, const int64_t* mrp, const int32_t* mcol,
         case T_I32: mxm_i32(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
         case T_I64: mxm_i64(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
         case T_BOOL: mxm_u8(n, mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul, out); return 0;
+    }
+    return -1;
+}
+
+/* C = A (+.*) B without a mask: SpMSpM_unmasked_ikj                         */
+/* (src/backend/spmspm/spmspm_hash_based_gustavson.h:9-153).  Row i folds,   */
+/* for every entry A(i,k) in row order and every B(k,j) in row order,        */
+/* C(i,j) = add(C(i,j), mul(A(i,k), B(k,j))) from the identity; entries whose */
+/* value is 0 are dropped.  Columns are written ascending (the reference     */
+/* writes them in hash-map order).  Returns nnz(C); writes at most cap       */
+/* entries.                                                                  */
+static int cmp_i32(const void* a, const void* b) {
+    const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+#define ORC_MXMU(SUF, T)                                                                                    \
+    static int64_t mxmu_##SUF(int64_t n, const int64_t* arp, const int32_t* acol, const T* aval,           \
+                              const int64_t* brp, const int32_t* bcol, const T* bval, int add, int mul,     \
+                              int64_t* crp, int32_t* ccol, T* cval, int64_t cap) {                          \
+        T* acc = (T*)malloc(sizeof(T) * (size_t)(n > 0 ? n : 1));                                           \
+        char* seen = (char*)calloc((size_t)(n > 0 ? n : 1), 1);                                             \
+        int32_t* touched = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));                     \
+        int64_t nz = 0;                                                                                     \
+        crp[0] = 0;                                                                                         \
+        for (int64_t i = 0; i < n; ++i) {                                                                   \
+            int64_t nt = 0;                                                                                 \
+            for (int64_t e = arp[i]; e < arp[i + 1]; ++e) {                                                 \
+                const int64_t k = acol[e];                                                                  \
+                for (int64_t q = brp[k]; q < brp[k + 1]; ++q) {                                             \
+                    const int32_t j = bcol[q];                                                              \
+                    if (!seen[j]) {                                                                         \
+                        seen[j] = 1;                                                                        \
+                        acc[j] = add_id_##SUF(add);                                                         \
+                        touched[nt++] = j;                                                                  \
+                    }                                                                                       \
+                    acc[j] = add_##SUF(add, acc[j], mul_##SUF(mul, aval[e], bval[q]));                      \
+                }                                                                                           \
+            }                                                                                               \
+            qsort(touched, (size_t)nt, sizeof(int32_t), cmp_i32);                                           \
+            for (int64_t t = 0; t < nt; ++t) {                                                              \
+                const int32_t j = touched[t];                                                               \
+                seen[j] = 0;                                                                                \
+                if (acc[j] != (T)0) {                                                                       \
+                    if (nz < cap) {                                                                         \
+                        ccol[nz] = j;                                                                       \
+                        cval[nz] = acc[j];                                                                  \
+                    }                                                                                       \
+                    ++nz;                                                                                   \
+                }                                                                                           \
+            }                                                                                               \
+            crp[i + 1] = nz;                                                                                \
+        }                                                                                                   \
+        free(acc);                                                                                          \
+        free(seen);                                                                                         \
+        free(touched);                                                                                      \
+        return nz;                                                                                          \
+    }
+ORC_MXMU(f32, float)
+ORC_MXMU(f64, double)
+ORC_MXMU(i32, int32_t)
+ORC_MXMU(i64, int64_t)
+ORC_MXMU(u8, uint8_t)
+
+int64_t orc_mxm_unmasked(int type, int64_t n, const int64_t* arp, const int32_t* acol, const void* aval,
+                         const int64_t* brp, const int32_t* bcol, const void* bval, int add, int mul, int64_t* crp,
+                         int32_t* ccol, void* cval, int64_t cap) {
+    switch (type) {
+        case T_F32: return mxmu_f32(n, arp, acol, aval, brp, bcol, bval, add, mul, crp, ccol, cval, cap);
+        case T_F64: return mxmu_f64(n, arp, acol, aval, brp, bcol, bval, add, mul, crp, ccol, cval, cap);
+        case T_I32: return mxmu_i32(n, arp, acol, aval, brp, bcol, bval, add, mul, crp, ccol, cval, cap);
+        case T_I64: return mxmu_i64(n, arp, acol, aval, brp, bcol, bval, add, mul, crp, ccol, cval, cap);
+        case T_BOOL: return mxmu_u8(n, arp, acol, aval, brp, bcol, bval, add, mul, crp, ccol, cval, cap);
     }
     return -1;
 }

@@ -270,6 +270,26 @@ def mxm_masked(mrp, mcol, arp, acol, aval, brp, bcol, bval, add, mul):
     return out[:len(mcol)]
 
 
+def mxm_unmasked(arp, acol, aval, brp, bcol, bval, add, mul):
+    """C = A (+.*) B without a mask (the unmasked IKJ product): (row_ptr, col, val),
+    columns ascending, entries whose value is 0 dropped."""
+    arp, acol = _csr(arp, acol)
+    brp, bcol = _csr(brp, bcol)
+    aval = np.ascontiguousarray(aval)
+    bval = np.ascontiguousarray(bval, dtype=aval.dtype)
+    n = len(arp) - 1
+    cap = int(np.diff(brp)[acol].sum()) if len(acol) else 0
+    crp = np.zeros(n + 1, dtype=np.int64)
+    ccol = np.zeros(max(cap, 1), dtype=np.int32)
+    cval = np.zeros(max(cap, 1), dtype=aval.dtype)
+    f = lib().orc_mxm_unmasked
+    f.restype = C.c_int64
+    nz = int(f(TYPE_OF[aval.dtype], C.c_int64(n), _p(arp), _p(acol), _p(aval), _p(brp), _p(bcol), _p(bval),
+               add, mul, _p(crp), _p(ccol), _p(cval), C.c_int64(cap)))
+    assert 0 <= nz <= cap
+    return crp, ccol[:nz], cval[:nz]
+
+
 def masked_count(mrp, mcol, arp, acol, brp, bcol):
     """Sum over the entries of C<M> = A (plus.one) B."""
     mrp, mcol = _csr(mrp, mcol)

@@ -351,6 +351,81 @@ def test_oracle_mxm_masked_matches_dense_products():
     assert np.array_equal(got.astype(np.int64), want)
 
 
+def _sp_csr(S):
+    S = S.tocsr()
+    S.sort_indices()
+    return S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data
+
+
+def test_oracle_mxm_unmasked_matches_scipy():
+    """int64 plus.times against scipy's sparse product; values in [-3, 3]
+    cancel, and the oracle drops the entries whose sum is 0 (as scipy's
+    eliminate_zeros does)."""
+    import scipy.sparse as sp
+    from oracle import oracle as O
+    rng = np.random.default_rng(11)
+    for n, dens in ((200, 0.03), (57, 0.2)):
+        A = sp.random(n, n, density=dens, random_state=rng, format="csr",
+                      data_rvs=lambda k: rng.integers(-3, 4, k)).astype(np.int64)
+        B = sp.random(n, n, density=dens, random_state=rng, format="csr",
+                      data_rvs=lambda k: rng.integers(-3, 4, k)).astype(np.int64)
+        A.eliminate_zeros()
+        B.eliminate_zeros()
+        arp, acol, av = _sp_csr(A)
+        brp, bcol, bv = _sp_csr(B)
+        crp, ccol, cval = O.mxm_unmasked(arp, acol, av, brp, bcol, bv, 0, 0)
+        W = (A @ B).tocsr()
+        W.eliminate_zeros()
+        wrp, wcol, wv = _sp_csr(W)
+        assert np.array_equal(crp, wrp) and np.array_equal(ccol, wcol) and np.array_equal(cval, wv)
+        cancelled = (A.astype(bool).astype(np.int64) @ B.astype(bool).astype(np.int64)).nnz - W.nnz
+        assert cancelled > 0  # some sums did cancel to 0 and were dropped
+
+
+def test_oracle_mxm_unmasked_identity_and_minplus():
+    """A (+.*) I = I (+.*) A = A (fp32 values, one term per entry); min.plus
+    against a brute-force minimum over k on small matrices."""
+    from oracle import oracle as O
+    n = 50
+    da, arp, acol = _dense_csr(n, 0.2, 12)
+    av = np.random.default_rng(13).random(len(acol)).astype(np.float32) + 0.5
+    irp = np.arange(n + 1, dtype=np.int64)
+    icol = np.arange(n, dtype=np.int32)
+    one = np.ones(n, np.float32)
+    for args in ((arp, acol, av, irp, icol, one), (irp, icol, one, arp, acol, av)):
+        crp, ccol, cval = O.mxm_unmasked(*args, 0, 0)
+        assert np.array_equal(crp, arp) and np.array_equal(ccol, acol) and np.array_equal(cval, av)
+    db, brp, bcol = _dense_csr(n, 0.2, 14)
+    rng = np.random.default_rng(15)
+    ai = rng.integers(1, 50, len(acol)).astype(np.int32)
+    bi = rng.integers(1, 50, len(bcol)).astype(np.int32)
+    big = np.iinfo(np.int64).max // 4
+    A = np.full((n, n), big, np.int64); A[da] = ai
+    B = np.full((n, n), big, np.int64); B[db] = bi
+    mp = np.min(A[:, :, None] + B[None, :, :], axis=1)
+    hit = (da.astype(np.int64) @ db.astype(np.int64)) > 0
+    crp, ccol, cval = O.mxm_unmasked(arp, acol, ai, brp, bcol, bi, 2, 4)
+    wi, wj = np.nonzero(hit)
+    assert np.array_equal(crp, np.concatenate([[0], np.cumsum(hit.sum(1))]))
+    assert np.array_equal(ccol, wj) and np.array_equal(cval.astype(np.int64), mp[wi, wj])
+
+
+def test_oracle_mxm_unmasked_folds_in_ikj_order():
+    """C(0,0) = ((0 + 1e8) + 1) + (-1e8) in A's row order: 0 in fp32 (the
+    entry is dropped), 1 in fp64 (kept)."""
+    from oracle import oracle as O
+    arp = np.array([0, 3, 3, 3], np.int64)
+    acol = np.array([0, 1, 2], np.int32)
+    brp = np.array([0, 1, 2, 3], np.int64)
+    bcol = np.array([0, 0, 0], np.int32)
+    for dt, want_nnz in ((np.float32, 0), (np.float64, 1)):
+        av = np.array([1e8, 1.0, -1e8], dt)
+        crp, ccol, cval = O.mxm_unmasked(arp, acol, av, brp, bcol, np.ones(3, dt), 0, 0)
+        assert crp[-1] == want_nnz
+        if want_nnz:
+            assert ccol[0] == 0 and cval[0] == 1.0
+
+
 def test_oracle_masked_count_is_the_triangle_count():
     import scipy.sparse as sp
     from oracle import oracle as O

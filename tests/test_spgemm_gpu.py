@@ -1,6 +1,8 @@
-"""Masked SpGEMM (C<M> = A (+.*) B) and triangle counting on the GPU against
-the oracle's masked IKJ product (pinned by dense products) and against
-trace(A^3) / 6 computed by scipy."""
+"""SpGEMM on the GPU: the masked product C<M> = A (+.*) B against the oracle's
+masked IKJ product (pinned by dense products), the unmasked product against
+the oracle's unmasked IKJ product (pinned by scipy, A I = A and brute-force
+min.plus) bit for bit, fp32 included (the device keeps the IKJ fold order),
+and triangle counting against trace(A^3) / 6 computed by scipy."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -60,14 +62,14 @@ def test_mxm_masked_matches_oracle(gbk, oracle, dt, sr, exact):
     assert np.count_nonzero(want) > 100  # the mask meets the product
 
 
-def test_mxm_rejects_complemented_and_missing_mask(gbk):
+def test_mxm_rejects_complemented_mask_and_aliasing(gbk):
     rp, col = gbk.gen_rmat(8, 4, seed=44)
     A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.int32))
     Cm = gbk.Matrix(gbk.INT32)
     with pytest.raises(gbk.GbkError, match="complementary mask"):
         gbk.mxm(Cm, A, gbk.ACC_NONE, "PlusMultiplies", A, A, desc_mask=gbk.STR_COMP)
-    with pytest.raises(gbk.GbkError):
-        gbk.mxm(Cm, None, gbk.ACC_NONE, "PlusMultiplies", A, A)
+    with pytest.raises(gbk.GbkError, match="aliases"):
+        gbk.mxm(A, None, gbk.ACC_NONE, "PlusMultiplies", A, A)
 
 
 @pytest.mark.parametrize("kind,scale", [("rmat", 12), ("rmat", 14), ("er", 12)])
@@ -92,3 +94,68 @@ def test_tricount_small_cases(gbk):
         r = np.array([e[0] for e in edges]); c = np.array([e[1] for e in edges])
         A.build_coo(r, c, np.ones(len(edges), np.int32))
         assert [gbk.tricount(A, m) for m in range(7)] == [want] * 7
+
+
+def _vals(dt, m, rng, signed=False):
+    if dt == np.uint8:
+        return (rng.random(m) < 0.8).astype(np.uint8)
+    if dt in (np.float32, np.float64):
+        return (rng.random(m) - (0.5 if signed else 0.0)).astype(dt)
+    lo = -3 if signed else 1
+    return rng.integers(lo, 50 if not signed else 4, m).astype(dt)
+
+
+@pytest.mark.parametrize("dt,sr", [(np.int32, "PlusMultiplies"), (np.int32, "MinimumPlus"),
+                                   (np.int64, "PlusMultiplies"), (np.float32, "PlusMultiplies"),
+                                   (np.float64, "PlusMultiplies"), (np.uint8, "LogicalOrAnd")])
+def test_mxm_unmasked_matches_oracle(gbk, oracle, dt, sr):
+    """C = A (+.*) B without a mask: structure, columns ascending and values
+    equal to the oracle's bit for bit; signed values make some sums cancel,
+    and those entries are dropped on both sides."""
+    add, mul = gbk.SEMIRINGS[sr]
+    rng = np.random.default_rng(51)
+    cases = [(gbk.gen_rmat(10, 8, seed=52), gbk.gen_rmat(10, 8, seed=53)),
+             (gbk.gen_uniform(11, 4, seed=54), gbk.gen_uniform(11, 4, seed=55))]
+    for (arp, acol), (brp, bcol) in cases:
+        signed = sr == "PlusMultiplies" and dt != np.uint8
+        av = _vals(dt, len(acol), rng, signed)
+        bv = _vals(dt, len(bcol), rng, signed)
+        A = gbk.Matrix.from_csr(arp, acol, av)
+        B = gbk.Matrix.from_csr(brp, bcol, bv)
+        Cm = gbk.Matrix(gbk.TYPE_OF[np.dtype(dt)])
+        gbk.mxm(Cm, None, gbk.ACC_NONE, sr, A, B)
+        crp, ccol, cval = Cm.export()
+        wrp, wcol, wval = oracle.mxm_unmasked(arp, acol, av, brp, bcol, bv, add, mul)
+        assert np.array_equal(crp, wrp) and np.array_equal(ccol, wcol), f"{dt.__name__} {sr}"
+        assert np.array_equal(cval, wval), f"{dt.__name__} {sr}"
+        assert len(wcol) > 1000
+        for M in (A, B, Cm):
+            M.free()
+
+
+def test_mxm_unmasked_edge_cases(gbk, oracle):
+    """An empty operand, a product with no terms, a result that replaces a
+    built C, and A (+.*) A on the special graphs (empty rows, a hub)."""
+    from tests.graphs import special_graphs
+    n = 64
+    Z = gbk.Matrix.from_csr(np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    rp, col = gbk.gen_rmat(6, 4, seed=56)
+    A = gbk.Matrix.from_csr(rp, col, np.ones(len(col), np.int32))
+    Cm = gbk.Matrix(gbk.INT32)
+    gbk.mxm(Cm, None, gbk.ACC_NONE, "PlusMultiplies", A, A)
+    first = Cm.export()
+    assert first[0][-1] > 0
+    for X, Y in ((Z, A), (A, Z), (Z, Z)):
+        gbk.mxm(Cm, None, gbk.ACC_NONE, "PlusMultiplies", X, Y)
+        crp, ccol, _ = Cm.export()
+        assert crp[-1] == 0 and len(ccol) == 0 and len(crp) == n + 1
+    for name, (grp, gcol) in special_graphs().items():
+        if len(gcol) == 0:
+            continue
+        v = (np.arange(len(gcol)) % 5 + 1).astype(np.int32)
+        G = gbk.Matrix.from_csr(grp, gcol, v)
+        gbk.mxm(Cm, None, gbk.ACC_NONE, "PlusMultiplies", G, G)
+        crp, ccol, cval = Cm.export()
+        wrp, wcol, wval = oracle.mxm_unmasked(grp, gcol, v, grp, gcol, v, 0, 0)
+        assert np.array_equal(crp, wrp) and np.array_equal(ccol, wcol) and np.array_equal(cval, wval), name
+        G.free()
