@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = {
     "csr": "k_spmv_merge + k_spmv_merge_fixup",
     "csr_seg": "k_hot_x + k_seg_flag (persistent, XCD-mapped segments, hot x in LDS, flagged pair ends) + "
-               "k_flag_fix + k_seg_merge8",
+               "k_flag_fix + k_seg_merge_rows",
     "lav": "k_gather + k_spmv_merge (XCD-mapped segment tiles) + k_spmv_merge_fixup + k_seg_merge",
     "sell_c": "k_sell_c + long-row k_spmv_merge + k_spmv_merge_fixup",
 }

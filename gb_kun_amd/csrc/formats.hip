@@ -70,8 +70,10 @@ struct SegCSR {
     int64_t npairs = 0;
     DevCSR v;  // virtual CSR: one row per (segment, row) pair
     int32_t* tile_map = nullptr;
-    int32_t* rpp = nullptr;
+    int32_t* rpp = nullptr;        // row -> its pairs (rpp, row_pairs), for the plain kernels
     int32_t* row_pairs = nullptr;
+    int32_t* pair_row = nullptr;   // pair -> row, and for every block of MERGE_ROWS rows and
+    int32_t* block_pair = nullptr; // segment its first pair (the per-row merge of CSR_SEG)
     void* partial = nullptr;
     int64_t n = 0;
     SegHot hot;  // built for CSR_SEG (hot.fcol == nullptr otherwise)
@@ -488,6 +490,8 @@ static void free_seg_fields(SegCSR& s) {
     dev_free(s.tile_map);
     dev_free(s.rpp);
     dev_free(s.row_pairs);
+    dev_free(s.pair_row);
+    dev_free(s.block_pair);
     dev_free(s.partial);
     dev_free(s.hot.hcol);
     dev_free(s.hot.hot_ids);
@@ -686,8 +690,7 @@ static int seg_pairs_device(const DevCSR& m, int type, int64_t nseg, const uint3
     int32_t* pair_row = nullptr;
     GBK_TRY(pair_seg.alloc(4 * (npairs > 0 ? npairs : 1)));
     GBK_TRY(dev_alloc((void**)&pair_row, 4 * (npairs > 0 ? npairs : 1)));
-    TmpBuf pair_row_buf;
-    pair_row_buf.p = pair_row;  // released at the end of the build
+    s.pair_row = pair_row;  // owned by the format from here on
     s.v.nrows = s.v.ncols = npairs;
     s.v.nnz = nnz;
     GBK_TRY(dev_alloc((void**)&s.v.row_ptr, 4 * (npairs + 1)));
@@ -740,6 +743,40 @@ static int seg_plain_tiles(SegCSR& s, int type, const std::vector<int32_t>& seg_
     GBK_TRY(d2h(tiles.data(), s.v.tiles, sizeof(int2) * (s.v.ntiles + 1), stream()));
     std::vector<int64_t> sp(seg_pair.begin(), seg_pair.end());
     return upload(&s.tile_map, xcd_tile_map(tiles, sp, s.nseg));
+}
+
+constexpr int MERGE_THREADS = 256;
+constexpr int MERGE_RPT = 4;                               // rows per thread
+constexpr int MERGE_ROWS = MERGE_THREADS * MERGE_RPT;      // rows per workgroup of the per-row merge
+
+// block_pair[b * nseg + g] = first pair of segment g whose row is at least
+// b * MERGE_ROWS (pairs are in row order inside a segment), b in [0, nblocks].
+__global__ void k_block_pairs(const int32_t* __restrict__ pair_row, const int32_t* __restrict__ seg_pair, int nseg,
+                              int64_t nblocks, int32_t* __restrict__ block_pair) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nblocks + 1) * nseg) return;
+    const int64_t b = t / nseg;
+    const int g = (int)(t % nseg);
+    const int64_t row = b * MERGE_ROWS;
+    int64_t lo = seg_pair[g], hi = seg_pair[g + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)pair_row[mid] < row) lo = mid + 1;
+        else hi = mid;
+    }
+    block_pair[t] = (int32_t)lo;
+}
+
+static int build_block_pairs(SegCSR& s) {
+    const int64_t nblocks = (s.n + MERGE_ROWS - 1) / MERGE_ROWS;
+    const int64_t m = (nblocks + 1) * s.nseg;
+    GBK_TRY(dev_alloc((void**)&s.block_pair, 4 * (m > 0 ? m : 1)));
+    if (m > 0) {
+        hipLaunchKernelGGL(k_block_pairs, grid_e(m), dim3(256), 0, stream(), s.pair_row, s.hot.seg_pair, s.nseg,
+                           nblocks, s.block_pair);
+        GBK_LAUNCH_CHECK();
+    }
+    return GBK_SUCCESS;
 }
 
 constexpr int FLAG_CHUNK = 512;  // entries per flagged tile (8 per lane of a wave)
@@ -899,10 +936,16 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
     if (n <= (int64_t)0x40000000) {
         // the flagged layout (columns below 2^30) replaces the column arrays
         GBK_TRY(build_flagged(s, vs, seg_pair, seg_k));
+        GBK_TRY(build_block_pairs(s));
+        GBK_HIP(hipStreamSynchronize(st));
         dev_free(o.hcol);
         o.hcol = nullptr;
         dev_free(s.v.col);
         s.v.col = nullptr;
+        dev_free(s.rpp);
+        s.rpp = nullptr;
+        dev_free(s.row_pairs);
+        s.row_pairs = nullptr;
     } else {
         dev_free(o.hcol);
         o.hcol = nullptr;
@@ -941,28 +984,56 @@ __global__ void k_seg_merge(const int32_t* __restrict__ rpp, const int32_t* __re
     y[r] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[r], acc);
 }
 
-// The same merge when a row has at most 8 pairs (8 segments): the pair ids
-// and then the sums are loaded unconditionally (clamped to the row's last
-// pair), so each row costs two rounds of loads instead of a chain.
+// Per-row merge by blocks of MERGE_ROWS rows.  A segment has at most one
+// pair per row, so a block's range of a segment holds at most MERGE_ROWS
+// pairs: each thread loads its MERGE_RPT pairs of the range of each of (up
+// to) 8 segments at once (coalesced, all in flight together), then the block
+// adds them into the LDS slots of their rows segment by segment, in segment
+// order: the same fold as k_seg_merge, without the row -> pair indirection
+// and its dependent loads.
 template <typename T, int ADD>
-__global__ void k_seg_merge8(const int32_t* __restrict__ rpp, const int32_t* __restrict__ row_pairs,
-                             const T* __restrict__ partial, T* __restrict__ y, int64_t n, int accum) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int p0 = rpp[r], cnt = rpp[r + 1] - p0;
-    T acc = add_identity<ADD, T>();
-    if (cnt > 0) {
-        int pid[8];
+__global__ __launch_bounds__(MERGE_THREADS) void k_seg_merge_rows(const int32_t* __restrict__ pair_row,
+                                                                  const T* __restrict__ partial,
+                                                                  const int32_t* __restrict__ block_pair, int nseg,
+                                                                  int64_t n, T* __restrict__ y, int accum) {
+    __shared__ T acc[MERGE_ROWS];
+    const int64_t r0 = (int64_t)blockIdx.x * MERGE_ROWS;
+    const T ident = add_identity<ADD, T>();
 #pragma unroll
-        for (int k = 0; k < 8; ++k) pid[k] = row_pairs[p0 + min(k, cnt - 1)];
-        T v[8];
+    for (int u = 0; u < MERGE_RPT; ++u) acc[threadIdx.x + u * MERGE_THREADS] = ident;
+    const int32_t* bp = block_pair + (int64_t)blockIdx.x * nseg;
+    for (int g0 = 0; g0 < nseg; g0 += 8) {
+        int rr[8][MERGE_RPT];
+        T pv[8][MERGE_RPT];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = partial[pid[k]];
+        for (int q = 0; q < 8; ++q) {
+            const int g = g0 + q;
+            const int lo = g < nseg ? bp[g] : 0, hi = g < nseg ? bp[nseg + g] : 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (k < cnt) acc = add_op<ADD, T>(acc, v[k]);
+            for (int u = 0; u < MERGE_RPT; ++u) {
+                const int p = lo + (int)threadIdx.x + u * MERGE_THREADS;
+                const bool ok = p < hi;
+                const int pc = ok ? p : 0;  // an in-range index for the unused lanes
+                const int pr = pair_row[pc];
+                rr[q][u] = ok ? pr - (int)r0 : -1;
+                pv[q][u] = partial[pc];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+#pragma unroll
+            for (int u = 0; u < MERGE_RPT; ++u)
+                if (rr[q][u] >= 0) acc[rr[q][u]] = add_op<ADD, T>(acc[rr[q][u]], pv[q][u]);
+            __syncthreads();
+        }
     }
-    y[r] = (accum == GBK_ACC_NONE) ? acc : binop<T>(accum, y[r], acc);
+#pragma unroll
+    for (int u = 0; u < MERGE_RPT; ++u) {
+        const int i = threadIdx.x + u * MERGE_THREADS;
+        const int64_t r = r0 + i;
+        if (r < n) y[r] = (accum == GBK_ACC_NONE) ? acc[i] : binop<T>(accum, y[r], acc[i]);
+    }
 }
 
 template <typename T> __device__ __forceinline__ T hot_shfl_up(T v, int off) {
@@ -1183,12 +1254,9 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
         GBK_LAUNCH_CHECK();
     }
     if (s->n > 0) {
-        if (s->nseg <= 8)
-            hipLaunchKernelGGL((k_seg_merge8<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st,
-                               s->rpp, s->row_pairs, (const T*)s->partial, y, s->n, accum);
-        else
-            hipLaunchKernelGGL((k_seg_merge<T, ADD>), dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st,
-                               s->rpp, s->row_pairs, (const T*)s->partial, y, s->n, accum);
+        hipLaunchKernelGGL((k_seg_merge_rows<T, ADD>), dim3((unsigned)((s->n + MERGE_ROWS - 1) / MERGE_ROWS)),
+                           dim3(MERGE_THREADS), 0, st, s->pair_row, (const T*)s->partial, s->block_pair, s->nseg, s->n,
+                           y, accum);
         GBK_LAUNCH_CHECK();
     }
     return GBK_SUCCESS;
@@ -1305,6 +1373,8 @@ static int build_lav_device(const DevCSR& m, const int32_t* optr, int type, doub
         }
         GBK_TRY(seg_pairs_device(m, type, nseg, ekey.as<uint32_t>(), o2n.as<int32_t>(), l.seg, seg_pair));
     }
+    dev_free(l.seg.pair_row);
+    l.seg.pair_row = nullptr;
     return seg_plain_tiles(l.seg, type, seg_pair);
 }
 
@@ -1413,8 +1483,14 @@ int64_t format_bytes(gbk_matrix A) {
             b += (int64_t)(s->padded * (4 + vs) + s->nchunks * (8 + 4 + 8 * s->C) + (s->nproblem + s->nlong) * 4 +
                            s->lng.nnz * (4 + vs) + s->lng.nrows * 4);
         const SegCSR* sg = A->seg[t] ? A->seg[t] : (A->lav[t] ? &A->lav[t]->seg : nullptr);
-        if (sg) b += (int64_t)(sg->npairs * (4 + 4 + vs + 4) + sg->v.nnz * vs + sg->n * 4 + sg->v.ntiles * 28);
-        if (sg && sg->v.col) b += sg->v.nnz * 4;
+        if (sg) {
+            b += (int64_t)(sg->npairs * (4 + vs) + sg->v.nnz * vs + sg->v.ntiles * 28);
+            if (sg->v.col) b += sg->v.nnz * 4;
+            if (sg->rpp) b += (sg->n + 1) * 4;
+            if (sg->row_pairs) b += sg->npairs * 4;
+            if (sg->pair_row) b += sg->npairs * 4;
+            if (sg->block_pair) b += ((sg->n + MERGE_ROWS - 1) / MERGE_ROWS + 1) * sg->nseg * 4;
+        }
         if (sg && sg->hot.fcol)
             b += (int64_t)(sg->v.nnz * 4 + (int64_t)sg->nseg * sg->hot.H * (4 + vs) + sg->hot.nftiles * (8 + 2 * vs) +
                            (sg->nseg + 1) * 16);
