@@ -55,8 +55,8 @@ struct SegHot {
     void* xh = nullptr;           // nseg * H gathered x values, per call
     int32_t* seg_pair = nullptr;  // nseg + 1: first pair of each segment
     // flagged-entry layout (k_seg_flag): fcol[e] = (e ends its pair) << 31 |
-    // (hot ? 1 << 30 | slot : column), padded with zeros to whole 512-entry
-    // chunks; a tile is one chunk's entries of one segment, ftiles[t] =
+    // (hot ? 1 << 30 | slot : column), padded with zeros to whole
+    // FLAG_CHUNK-entry chunks; a tile is one chunk's entries of one segment, ftiles[t] =
     // {first entry, first pair | (that pair began in an earlier tile) << 31}.
     int32_t* fcol = nullptr;
     int64_t nftiles = 0;
@@ -545,6 +545,11 @@ static int64_t seg_cols_default(int64_t n) {
 // Persistent hot-column kernel shape: 16 waves per block, the block's LDS
 // holding the hot x values of its segment.
 constexpr int HOT_WAVES = 16;
+// Flagged tiles: FLAG_EPL consecutive entries per lane of a wave.
+constexpr int FLAG_EPL = 4;
+constexpr int FLAG_CHUNK = 64 * FLAG_EPL;
+constexpr int FLAG_CNT_BITS = 3;  // bits of a lane's flag count (0..FLAG_EPL)
+static_assert(FLAG_EPL <= 16 && (1 << FLAG_CNT_BITS) > FLAG_EPL, "flag counts fit the ballots");
 
 // LDS slots per segment for values of vs bytes: what one workgroup's LDS
 // holds, a multiple of 64.
@@ -695,9 +700,9 @@ static int seg_pairs_device(const DevCSR& m, int type, int64_t nseg, const uint3
     s.v.nnz = nnz;
     GBK_TRY(dev_alloc((void**)&s.v.row_ptr, 4 * (npairs + 1)));
     GBK_TRY(dev_alloc((void**)&s.v.col, 4 * z1));
-    // values padded with zeros to whole 512-entry chunks (the flagged
-    // kernel loads whole chunks)
-    const int64_t zpad = (nnz + 511) / 512 * 512 + 512;
+    // values padded with zeros to whole chunks (the flagged kernel loads
+    // whole chunks)
+    const int64_t zpad = (nnz + FLAG_CHUNK - 1) / FLAG_CHUNK * FLAG_CHUNK + FLAG_CHUNK;
     GBK_TRY(dev_alloc(&s.v.val, vs * zpad));
     GBK_HIP(hipMemsetAsync((char*)s.v.val + vs * nnz, 0, vs * (zpad - nnz), st));
     if (nnz > 0) {
@@ -779,7 +784,6 @@ static int build_block_pairs(SegCSR& s) {
     return GBK_SUCCESS;
 }
 
-constexpr int FLAG_CHUNK = 512;  // entries per flagged tile (8 per lane of a wave)
 
 __global__ void k_flag_encode(const int32_t* __restrict__ hcol, int64_t nnz, int64_t padded,
                               int32_t* __restrict__ fcol) {
@@ -1055,19 +1059,32 @@ __global__ void k_hot_x(const int32_t* __restrict__ ids, const T* __restrict__ x
     }
 }
 
-// Eight consecutive elements from an address aligned to their size.
+// FLAG_EPL consecutive elements from an address aligned to their size.
 typedef unsigned int gbk_u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int gbk_u32x2 __attribute__((ext_vector_type(2)));
-template <typename V> __device__ __forceinline__ void load8_nt(const V* __restrict__ p, V (&out)[8]) {
-    if constexpr (sizeof(V) == 1) {
-        const gbk_u32x2 u = __builtin_nontemporal_load((const gbk_u32x2*)p);
-        __builtin_memcpy(out, &u, 8);
-    } else {
-        constexpr int NV = (int)(8 * sizeof(V) / 16);
-        gbk_u32x4 u[NV];
+template <typename V> __device__ __forceinline__ void loadn_nt(const V* __restrict__ p, V (&out)[FLAG_EPL]) {
+    constexpr int B = FLAG_EPL * (int)sizeof(V);
+    if constexpr (B % 4 != 0) {
+        static_assert(B % 2 == 0, "whole 2-byte pieces");
+        unsigned short u[B / 2];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) u[i] = __builtin_nontemporal_load((const gbk_u32x4*)p + i);
-        __builtin_memcpy(out, u, 8 * sizeof(V));
+        for (int i = 0; i < B / 2; ++i) u[i] = __builtin_nontemporal_load((const unsigned short*)p + i);
+        __builtin_memcpy(out, u, B);
+    } else if constexpr (B % 8 != 0) {
+        unsigned int u[B / 4];
+#pragma unroll
+        for (int i = 0; i < B / 4; ++i) u[i] = __builtin_nontemporal_load((const unsigned int*)p + i);
+        __builtin_memcpy(out, u, B);
+    } else if constexpr (B % 16 != 0) {
+        gbk_u32x2 u[B / 8];
+#pragma unroll
+        for (int i = 0; i < B / 8; ++i) u[i] = __builtin_nontemporal_load((const gbk_u32x2*)p + i);
+        __builtin_memcpy(out, u, B);
+    } else {
+        gbk_u32x4 u[B / 16];
+#pragma unroll
+        for (int i = 0; i < B / 16; ++i) u[i] = __builtin_nontemporal_load((const gbk_u32x4*)p + i);
+        __builtin_memcpy(out, u, B);
     }
 }
 
@@ -1076,10 +1093,10 @@ template <typename V> __device__ __forceinline__ void load8_nt(const V* __restri
 // serves segments b % 8, b % 8 + 8, ...; within a segment, wave w of block j
 // takes tiles fseg_tile[s] + j * HOT_WAVES + w in steps of (grid / 8) *
 // HOT_WAVES.  Every (segment, row) pair has at least one entry, so a pair
-// ends exactly at an entry whose flag is set and tiles need no row items.  Lane L of a wave folds the
-// eight entries chunk + 8L .. 8L + 7 in order; its n-th flag ends pair
-// first_pair + (flags of lanes below L) + n, the prefix taken from four
-// ballots of the per-lane counts.  A lane's sum up to its first flag is
+// ends exactly at an entry whose flag is set and tiles need no row items.
+// Lane L of a wave folds the FLAG_EPL entries chunk + FLAG_EPL L ... in
+// order; its n-th flag ends pair first_pair + (flags of lanes below L) + n,
+// the prefix taken from one ballot per bit of the per-lane counts.  A lane's sum up to its first flag is
 // joined with the lanes below by a wave64 segmented scan; the tile's trailing
 // sum and a deferred first pair go to carry (k_flag_fix).
 template <typename T, int ADD, int MUL>
@@ -1105,12 +1122,12 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
             const int chunk0 = ftiles[t_lo].x / FLAG_CHUNK;
             int2 ti = ftiles[tile];
             int k1 = ftiles[tile + 1].x;
-            int c[8];
-            T a[8];
+            int c[FLAG_EPL];
+            T a[FLAG_EPL];
             {
-                const int base = (chunk0 + (int)(tile - t_lo)) * FLAG_CHUNK + lane * 8;
-                load8_nt(fcol + base, c);
-                load8_nt(vval + base, a);
+                const int base = (chunk0 + (int)(tile - t_lo)) * FLAG_CHUNK + lane * FLAG_EPL;
+                loadn_nt(fcol + base, c);
+                loadn_nt(vval + base, a);
             }
             while (true) {
                 // the next tile (clamped to the last one): its descriptor
@@ -1119,40 +1136,40 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
                 const int64_t nt = min(tile + step, t_hi - 1);
                 const int2 ti_n = ftiles[nt];
                 const int k1_n = ftiles[nt + 1].x;
-                T xv[8];
+                T xv[FLAG_EPL];
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
+                for (int m = 0; m < FLAG_EPL; ++m) {
                     const int idx = c[m] & 0x3fffffff;
                     if (c[m] & 0x40000000) xv[m] = s_x[idx];
                     else xv[m] = x[(uint32_t)idx];
                 }
-                int cn[8];
-                T an[8];
+                int cn[FLAG_EPL];
+                T an[FLAG_EPL];
                 {
-                    const int base_n = (chunk0 + (int)(nt - t_lo)) * FLAG_CHUNK + lane * 8;
-                    load8_nt(fcol + base_n, cn);
-                    load8_nt(vval + base_n, an);
+                    const int base_n = (chunk0 + (int)(nt - t_lo)) * FLAG_CHUNK + lane * FLAG_EPL;
+                    loadn_nt(fcol + base_n, cn);
+                    loadn_nt(vval + base_n, an);
                 }
                 const int k0 = ti.x;
                 const int pfirst = ti.y & 0x7fffffff;
                 const bool defer0 = ti.y < 0;
-                const int base = (k0 & ~(FLAG_CHUNK - 1)) + lane * 8;
+                const int base = (k0 & ~(FLAG_CHUNK - 1)) + lane * FLAG_EPL;
                 // entries of this tile: [k0, k1) of the chunk
-                const int mlo = min(max(k0 - base, 0), 8), mhi = min(max(k1 - base, 0), 8);
+                const int mlo = min(max(k0 - base, 0), FLAG_EPL), mhi = min(max(k1 - base, 0), FLAG_EPL);
                 const uint32_t valid = ((1u << mhi) - 1u) & ~((1u << mlo) - 1u);
                 uint32_t fl = 0;
 #pragma unroll
-                for (int m = 0; m < 8; ++m) fl |= ((uint32_t)c[m] >> 31) << m;
+                for (int m = 0; m < FLAG_EPL; ++m) fl |= ((uint32_t)c[m] >> 31) << m;
                 fl &= valid;
                 const int cnt = __builtin_popcount(fl);
                 int before = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) before += __builtin_popcountll(__ballot((cnt >> q) & 1) & below) << q;
+                for (int q = 0; q < FLAG_CNT_BITS; ++q) before += __builtin_popcountll(__ballot((cnt >> q) & 1) & below) << q;
                 T acc = ident, firstsum = ident;
                 int nf = 0;
                 const int pbase = pfirst + before;
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
+                for (int m = 0; m < FLAG_EPL; ++m) {
                     const T pm = ((valid >> m) & 1u) ? mul_op<MUL, T>(a[m], xv[m]) : ident;
                     acc = add_op<ADD, T>(acc, pm);
                     if ((fl >> m) & 1u) {
@@ -1188,7 +1205,7 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
                 ti = ti_n;
                 k1 = k1_n;
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
+                for (int m = 0; m < FLAG_EPL; ++m) {
                     c[m] = cn[m];
                     a[m] = an[m];
                 }
