@@ -1217,23 +1217,69 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
 
 // Pairs that began in an earlier tile and end in tile t: the deferred part
 // plus the trailing sums of the tiles back to the one holding the pair start,
-// in tile order.
+// added from the nearest tile back.  A thread walks up to FIX_STEPS tiles
+// back itself (most pairs span one or two tiles); the walks that go further
+// (pairs of hub rows) are finished by the whole wave, one at a time, 64
+// earlier tiles per round (lane l: tile u - l), each round's sums combined by
+// a fixed butterfly, so the result is the same on every run.
+constexpr int FIX_STEPS = 8;
+
 template <typename T, int ADD>
-__global__ void k_flag_fix(const int32_t* __restrict__ vrp, const int2* __restrict__ ftiles, int64_t ntiles,
-                           const T* __restrict__ carry, T* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_flag_fix(const int32_t* __restrict__ vrp, const int2* __restrict__ ftiles,
+                                                  int64_t ntiles, const T* __restrict__ carry,
+                                                  T* __restrict__ partial) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    const int2 ti = ftiles[t];
-    if (ti.y >= 0) return;
-    const int p = ti.y & 0x7fffffff;
-    if (vrp[p + 1] > ftiles[t + 1].x) return;  // the pair does not end in this tile
-    const int start = vrp[p];
-    T total = carry[2 * t + 1];
-    for (int64_t u = t - 1; u >= 0; --u) {
-        total = add_op<ADD, T>(carry[2 * u], total);
-        if (ftiles[u].x <= start) break;
+    const int lane = threadIdx.x & 63;
+    const T ident = add_identity<ADD, T>();
+    bool open = false;  // the walk continues below tile `next`
+    int p = 0, start = 0;
+    int64_t next = 0;
+    T total = ident;
+    if (t < ntiles) {
+        const int2 ti = ftiles[t];
+        if (ti.y < 0) {
+            p = ti.y & 0x7fffffff;
+            if (vrp[p + 1] <= ftiles[t + 1].x) {  // the pair ends in this tile
+                start = vrp[p];
+                total = carry[2 * t + 1];
+                int64_t u = t - 1;
+                bool done = false;
+                for (int k = 0; k < FIX_STEPS && u >= 0; ++k, --u) {
+                    total = add_op<ADD, T>(carry[2 * u], total);
+                    if (ftiles[u].x <= start) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (done || u < 0) partial[p] = total;
+                else {
+                    open = true;
+                    next = u;
+                }
+            }
+        }
     }
-    partial[p] = total;
+    uint64_t todo = __ballot(open);
+    while (todo) {
+        const int src = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        const int st = __shfl(start, src, 64);
+        const int64_t hi0 = (int64_t)__shfl((int)next, src, 64);
+        T acc = hot_shfl(total, src);
+        for (int64_t hi = hi0; hi >= 0; hi -= 64) {
+            const int64_t u = hi - lane;
+            const bool in = u >= 0;
+            const int k0 = in ? ftiles[u].x : 0;
+            const uint64_t stop = __ballot(in && k0 <= st);  // lanes at or past the pair start
+            const int last = stop ? __ffsll((long long)stop) - 1 : 63;
+            T v = (in && lane <= last) ? carry[2 * u] : ident;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v = add_op<ADD, T>(v, hot_shfl(v, lane ^ off));
+            acc = add_op<ADD, T>(v, acc);
+            if (stop) break;
+        }
+        if (lane == src) partial[p] = acc;
+    }
 }
 
 template <typename T, int ADD, int MUL> static int hot_prepare() {
