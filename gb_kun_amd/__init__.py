@@ -143,6 +143,7 @@ _sig("gbk_msbfs", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
 _sig("gbk_msbfs_device", C.c_int, _vp, _pi64, C.c_int, _vp, _pi64)
 _sig("gbk_ppr_multi", C.c_int, _vp, _pi64, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double, C.c_double,
      C.POINTER(C.c_int))
+_sig("gbk_ppr_multi_device", C.c_int, _vp, _pi64, C.c_int, _vp, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
 _sig("gbk_sssp_multi", C.c_int, _vp, _pi64, C.c_int, _pi32, _pi64)
 _sig("gbk_graph_batch_new", C.c_int, C.POINTER(_vp), C.c_int, _pi64, C.POINTER(_pi64), C.POINTER(_pi32))
 _sig("gbk_graph_batch_free", C.c_int, _vp)
@@ -567,6 +568,16 @@ def ppr_multi(A, sources, itermax=100, damping=0.85, tol=1e-6):
     check(lib.gbk_ppr_multi(A.h, _ptr(src, C.c_int64), len(src), _ptr(r, C.c_float), itermax, damping, tol,
                             _ptr(it, C.c_int)))
     return r, it
+
+
+def ppr_multi_device(A, sources, ranks_ptr, itermax=100, damping=0.85, tol=1e-6):
+    """Personalized PageRank writing ranks[len(sources), n] (float) to a device array; returns the iterations
+    each column ran."""
+    src = np.ascontiguousarray(sources, dtype=np.int64)
+    it = np.zeros(len(src), dtype=np.int32)
+    check(lib.gbk_ppr_multi_device(A.h, _ptr(src, C.c_int64), len(src), _vp(ranks_ptr), itermax, damping, tol,
+                                   _ptr(it, C.c_int)))
+    return it
 
 
 def sssp_multi(A, sources):

@@ -80,18 +80,21 @@ __global__ __launch_bounds__(256) void k_ppr_sink(const float* __restrict__ R, i
 }
 
 // Rn = d * Y + teleport on the source rows; frozen columns keep R.  Adds the
-// L1 change of every column to delta[q].
+// L1 change of every column to delta[q] and, when sink_flag is given, the new
+// ranks of the sinks to sink_next[q] (the next iteration's sink sums, so that
+// no separate pass gathers the sink rows).
 // Rn may be Y (each element is read, then written, by the same thread).
 __global__ __launch_bounds__(256) void k_ppr_update(const float* __restrict__ R, const float* Y, float* Rn, int64_t n,
                                                     int k, int ld,
                                                     const int32_t* __restrict__ src, const double* __restrict__ sink,
                                                     const int32_t* __restrict__ done, float damping,
-                                                    double* __restrict__ delta) {
-    __shared__ double s[256];
+                                                    double* __restrict__ delta, const int32_t* __restrict__ sink_flag,
+                                                    double* __restrict__ sink_next) {
+    __shared__ double s[256], s2[256];
     const int per = 256 / k;
     const int q = threadIdx.x % k;
     const int j = threadIdx.x / k;
-    double acc = 0.0;
+    double acc = 0.0, sacc = 0.0;
     if (j < per) {
         const float tele = (float)((1.0 - (double)damping) + (double)damping * sink[q]);
         const int sq = src[q];
@@ -108,14 +111,20 @@ __global__ __launch_bounds__(256) void k_ppr_update(const float* __restrict__ R,
             }
             Rn[i] = nv;
             acc += fabs((double)nv - (double)old);
+            if (sink_flag && sink_flag[v]) sacc += (double)nv;
         }
     }
     s[threadIdx.x] = acc;
+    s2[threadIdx.x] = sacc;
     __syncthreads();
     if (threadIdx.x < k) {
-        double t = 0.0;
-        for (int jj = 0; jj < per; ++jj) t += s[jj * k + threadIdx.x];
+        double t = 0.0, t2 = 0.0;
+        for (int jj = 0; jj < per; ++jj) {
+            t += s[jj * k + threadIdx.x];
+            t2 += s2[jj * k + threadIdx.x];
+        }
         atomicAdd(delta + threadIdx.x, t);
+        if (sink_flag) atomicAdd(sink_next + threadIdx.x, t2);
     }
 }
 
@@ -147,7 +156,7 @@ struct PprLayout {
     int32_t *flag, *offs, *sinks;
     void* partial;
     size_t part_bytes;
-    double *sink, *delta;
+    double *sink, *sink2, *delta;
     int32_t* done;
     int32_t* src;
     size_t total;
@@ -170,6 +179,7 @@ static PprLayout ppr_layout(const DevCSR& m, int64_t n, int k, char* base) {
     L.part_bytes = scan_partial_bytes(n);
     L.partial = take(L.part_bytes);
     L.sink = (double*)take(sizeof(double) * 64);
+    L.sink2 = (double*)take(sizeof(double) * 64);
     L.delta = (double*)take(sizeof(double) * 64);
     L.done = (int32_t*)take(sizeof(int32_t) * 64);
     L.src = (int32_t*)take(sizeof(int32_t) * 64);
@@ -204,27 +214,33 @@ int ppr_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_by
     for (int q = 0; q < k; ++q) iters[q] = itermax;
     GBK_HIP(hipMemcpyAsync(L.done, done.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
     const int per = 256 / k;
-    const int64_t ub = std::min<int64_t>((n + per - 1) / per, (int64_t)device_cu_count() * 32);
+    // the update streams n k values: a few workgroups per CU suffice, and
+    // fewer of them add fewer per-column partial sums atomically
+    const int64_t ub = std::min<int64_t>((n + per - 1) / per, (int64_t)device_cu_count() * 4);
     const int64_t sb = std::min<int64_t>(((int64_t)nsinks + per - 1) / per, (int64_t)device_cu_count() * 8);
     std::vector<double> delta(k);
     float* R = L.R;
     float* Y = L.Y;
+    double* sink_cur = L.sink;
+    double* sink_next = L.sink2;
     int remaining = k;
     hipStream_t prev = stream();
     set_thread_stream(st);
+    // the sink sums of the starting ranks; each update forms the next ones
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, sink_cur, 64);
+    if (nsinks > 0 && sb > 0)
+        hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st, R, k, k, L.sinks, (int64_t)nsinks,
+                           sink_cur);
     int rc = GBK_SUCCESS;
     for (int it = 1; it <= itermax && remaining > 0; ++it) {
         if (k == 1) rc = spmv_merge_ws(m, GBK_FP32, R, Y, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L.carry);
         else rc = spmm_run(m, GBK_FP32, R, Y, k, GBK_ADD_PLUS, GBK_MUL_TIMES, GBK_ACC_NONE, L.carry, st);
         if (rc != GBK_SUCCESS) break;
-        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, L.sink, 64);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, sink_next, 64);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, L.delta, 64);
-        if (nsinks > 0 && sb > 0)
-            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st, R, k, k, L.sinks, (int64_t)nsinks,
-                               L.sink);
         // Y is overwritten with the new ranks, then R and Y swap roles
         hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)std::max<int64_t>(ub, 1)), dim3(256), 0, st, R, Y, Y, n, k, k,
-                           L.src, L.sink, L.done, (float)damping, L.delta);
+                           L.src, sink_cur, L.done, (float)damping, L.delta, L.flag, sink_next);
         if (hipGetLastError() != hipSuccess) {
             rc = fail(GBK_PANIC, "PageRank kernel launch failed");
             break;
@@ -235,6 +251,7 @@ int ppr_run(gbk_matrix A, const int32_t* src_host, int k, void* ws, size_t ws_by
             break;
         }
         std::swap(R, Y);
+        std::swap(sink_cur, sink_next);
         bool changed = false;
         for (int q = 0; q < k; ++q) {
             if (!done[q] && tol > 0.0 && delta[q] < tol) {
@@ -499,7 +516,7 @@ class PprEngine : public Engine {
             hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st_, L_.R, kw, k_, L_.sinks,
                                (int64_t)nsinks_, L_.sink);
         hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)ub), dim3(256), 0, st_, L_.R, L_.Y, L_.Y, n_, kw, k_, L_.src,
-                           L_.sink, L_.done, (float)damping_, L_.delta);
+                           L_.sink, L_.done, (float)damping_, L_.delta, (const int32_t*)nullptr, (double*)nullptr);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipMemcpyAsync(hdelta_, L_.delta, sizeof(double) * kw, hipMemcpyDeviceToHost, st_));
         GBK_HIP(hipStreamSynchronize(st_));
@@ -676,6 +693,24 @@ extern "C" int gbk_ppr_multi(gbk_matrix A, const int64_t* sources, int k, float*
     GBK_TRY(scratch(SLOT_MSBFS_OUT, sizeof(float) * (size_t)k * A->n, &out));
     GBK_TRY(ppr_run(A, s32.data(), k, ws, wb, (float*)out, itermax, damping, tol, iters, st));
     GBK_HIP(hipMemcpyAsync(ranks, out, sizeof(float) * (size_t)k * A->n, hipMemcpyDeviceToHost, st));
+    GBK_HIP(hipStreamSynchronize(st));
+    return GBK_SUCCESS;
+}
+
+extern "C" int gbk_ppr_multi_device(gbk_matrix A, const int64_t* sources, int k, float* ranks_dev, int itermax,
+                                    double damping, double tol, int* iters) {
+    if (!A) return fail(GBK_UNINITIALIZED_OBJECT, "null matrix");
+    if (!A->built) return fail(GBK_UNINITIALIZED_OBJECT, "matrix not built");
+    if (!ranks_dev || !iters) return fail(GBK_NULL_POINTER, "null output");
+    if (itermax < 1) return fail(GBK_INVALID_VALUE, "itermax must be positive");
+    std::vector<int32_t> s32;
+    GBK_TRY(check_sources(A, sources, k, s32));
+    GBK_TRY(ensure_init());
+    hipStream_t st = stream();
+    void* ws;
+    const size_t wb = ppr_ws_bytes(A, k);
+    GBK_TRY(scratch(SLOT_PPR, wb, &ws));
+    GBK_TRY(ppr_run(A, s32.data(), k, ws, wb, ranks_dev, itermax, damping, tol, iters, st));
     GBK_HIP(hipStreamSynchronize(st));
     return GBK_SUCCESS;
 }

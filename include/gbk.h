@@ -419,6 +419,9 @@ int gbk_msbfs_device(gbk_matrix A, const int64_t* sources, int nsrc, int32_t* le
  * of column q. */
 int gbk_ppr_multi(gbk_matrix A, const int64_t* sources, int k, float* ranks, int itermax,
                   double damping, double tol, int* iters);
+/* Same, writing the ranks (k*n floats, source-major) to a device array. */
+int gbk_ppr_multi_device(gbk_matrix A, const int64_t* sources, int k, float* ranks_dev, int itermax,
+                         double damping, double tol, int* iters);
 
 /* min.plus SSSP for k <= 64 sources at once on an INT32 weight matrix
  * (sssp_bellman_ford_blast, algorithms/sssp/sssp_blast.hpp:8-72, once per

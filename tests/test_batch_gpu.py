@@ -118,3 +118,28 @@ def test_ppr_multi_vs_fp64(gbk, oracle, scale, k):
         want, _ = oracle.ppr_f64(rp, col, vals, source=int(s), itermax=int(its[q]))
         l1 = np.abs(r[q].astype(np.float64) - want).sum()
         assert l1 <= 1e-6, f"source {s}: L1 {l1:.2e} after {its[q]} iterations"
+
+
+def test_ppr_multi_device_matches_host_entry(gbk, oracle):
+    """The device-output entry point leaves in HBM the ranks the host entry
+    point returns (same kernels), for k = 1 and k = 24 on a graph with sinks;
+    the sink sums the update pass forms for the next iteration match the
+    oracle's restatement of the recurrence."""
+    from gb_kun_amd.device import DeviceArray
+    rp, col = gbk.gen_rmat(13, 8, seed=112)
+    n = len(rp) - 1
+    vals = gbk.inv_coldeg_f32(n, col)
+    A = gbk.Matrix.from_csr(rp, col, vals)
+    assert np.count_nonzero(np.bincount(col, minlength=n) == 0) > 0
+    for k in (1, 24):
+        src = _sources(rp, k, 113)
+        r, its = gbk.ppr_multi(A, src, itermax=200, tol=1e-7)
+        d = DeviceArray((k, n), np.float32)
+        its_d = gbk.ppr_multi_device(A, src, d.ptr, itermax=200, tol=1e-7)
+        got = d.numpy()
+        d.free()
+        assert np.array_equal(its, its_d)
+        assert np.abs(got.astype(np.float64) - r).sum(axis=1).max() <= 1e-7
+        for q in (0, k - 1):
+            want, _ = oracle.ppr_f64(rp, col, vals, source=int(src[q]), itermax=int(its[q]))
+            assert np.abs(got[q].astype(np.float64) - want).sum() <= 1e-6
