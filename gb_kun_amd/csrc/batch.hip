@@ -304,6 +304,16 @@ static dim3 capped_grid(int64_t threads) {
 // bit is found.  Writes the new frontier word of every vertex (0 if none).
 // Grid-stride over vertex groups; the stride is a multiple of G, so a group
 // keeps one vertex per pass.
+// Lanes per vertex of the MS-BFS pull step for an average in-degree.  A
+// vertex stops only once every missing search bit is found, so it scans far
+// more of its list than a single search does; wide groups win (directed
+// R-MAT 22, 64 searches: G = 1: 16.2, 4: 8.5, 16: 5.9 ms).
+static int msbfs_group(double avg) {
+    int G = 4;
+    while (G < 32 && G < avg) G <<= 1;
+    return G;
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void k_msbfs_pull(const int32_t* __restrict__ crp, const int32_t* __restrict__ ccol,
                                                     int64_t n, uint64_t all, const uint64_t* __restrict__ F,
@@ -594,11 +604,7 @@ int msbfs_run(gbk_matrix A, const int32_t* src_dev, int nsrc, void* ws, size_t w
     hipLaunchKernelGGL(k_msbfs_sources, dim3(1), dim3(64), 0, st, src_dev, nsrc, F, V, lev);
     GBK_LAUNCH_CHECK();
 
-    int G = 4;
-    {
-        const double avg = n ? (double)in.nnz / (double)n : 0;
-        while (G < 32 && G < avg) G <<= 1;
-    }
+    const int G = msbfs_group(n ? (double)in.nnz / (double)n : 0);
     int64_t levels = 1, pushes = 0, pulls = 0, reached = nsrc, edges_scanned = 0;
     int32_t cur = 1;
     unsigned long long h[3];
@@ -743,10 +749,7 @@ class BfsEngine : public Engine {
         hipLaunchKernelGGL(k_zero_u64, grid_of(32 * n_), dim3(256), 0, st_, (uint64_t*)L_.lev, 32 * n_);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipStreamSynchronize(st_));
-        int G = 4;
-        const double avg = n_ ? (double)A_->csc.nnz / (double)n_ : 0;
-        while (G < 32 && G < avg) G <<= 1;
-        G_ = G;
+        G_ = msbfs_group(n_ ? (double)A_->csc.nnz / (double)n_ : 0);
         return GBK_SUCCESS;
     }
     int slots() const override { return 64; }
