@@ -212,16 +212,18 @@ __global__ __launch_bounds__(256) void k_lvl_compact(const int32_t* __restrict__
     }
 }
 
-// The frontier bitmap of a queue.
-__global__ void k_fb_clear(uint32_t* __restrict__ fb, int64_t words) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < words) fb[i] = 0;
-}
-__global__ void k_fb_set(uint32_t* __restrict__ fb, const int32_t* __restrict__ q, int64_t nq) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nq) {
-        const int u = q[i];
-        atomicOr(fb + (u >> 5), 1u << (u & 31));
+// The frontier bitmap: bit v set when level[v] == cur (the vertices of the
+// current queue).  One ballot per 64 vertices writes two whole words (no
+// atomics; setting bits from the queue made consecutive threads contend for
+// the same word).
+__global__ void k_fb_from_levels(const int32_t* __restrict__ level, int64_t n, int32_t cur,
+                                 uint32_t* __restrict__ fb, int64_t words) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t m = __ballot(v < n && level[v] == cur);
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t w = v >> 5;
+        if (w < words) fb[w] = (uint32_t)m;
+        if (w + 1 < words) fb[w + 1] = (uint32_t)(m >> 32);
     }
 }
 // Vertices with more than BFS_HEAVY in-edges.
@@ -288,7 +290,6 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
     hipLaunchKernelGGL(k_set_level, dim3(1), dim3(1), 0, st, level, source, s.q[0], s.cnt);
     GBK_LAUNCH_CHECK();
     int64_t nq = 1, m_u = A->nnz, levels = 0, pushes = 0, pulls = 0, traversed = 0;
-    int64_t nheavy = -1;  // listed on the first pull step
     int32_t cur = 1;
     bool pull = false;
     int qi = 0;
@@ -319,22 +320,25 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
         GBK_TRY(fill_value(GBK_INT32, ncnt, 1, 0.0));
         if (pull) {
             ++pulls;
-            if (nheavy < 0) {
+            if (A->bfs_nheavy < 0) {
+                // the heavy vertices depend on the graph only: listed once per matrix
                 GBK_TRY(fill_value(GBK_INT32, s.cnt + 2, 1, 0.0));
                 hipLaunchKernelGGL(k_heavy_list, grid_n(n), dim3(256), 0, st, in.row_ptr, n, heavy, s.heavy, s.cnt + 2);
                 GBK_LAUNCH_CHECK();
                 int32_t hc = 0;
                 GBK_HIP(hipMemcpyAsync(&hc, s.cnt + 2, sizeof(int32_t), hipMemcpyDeviceToHost, st));
                 GBK_HIP(hipStreamSynchronize(st));
-                nheavy = hc;
+                GBK_TRY(dev_alloc((void**)&A->bfs_heavy, sizeof(int32_t) * (hc > 0 ? hc : 1)));
+                if (hc > 0)
+                    GBK_HIP(hipMemcpyAsync(A->bfs_heavy, s.heavy, sizeof(int32_t) * hc, hipMemcpyDeviceToDevice, st));
+                A->bfs_nheavy = hc;
             }
-            hipLaunchKernelGGL(k_fb_clear, grid_n(fwords), dim3(256), 0, st, s.fb, fwords);
-            GBK_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_fb_set, grid_n(nq), dim3(256), 0, st, s.fb, s.q[qi], nq);
+            const int64_t nheavy = A->bfs_nheavy;
+            hipLaunchKernelGGL(k_fb_from_levels, grid_n(fwords * 32), dim3(256), 0, st, level, n, cur, s.fb, fwords);
             GBK_LAUNCH_CHECK();
             if (nheavy > 0) {
                 hipLaunchKernelGGL(k_bfs_pull_heavy, dim3((unsigned)nheavy), dim3(256), 0, st, in.row_ptr, in.col,
-                                   s.heavy, nheavy, level, s.fb, cur);
+                                   A->bfs_heavy, nheavy, level, s.fb, cur);
                 GBK_LAUNCH_CHECK();
             }
 #define GBK_PULL(GG)                                                                                  \

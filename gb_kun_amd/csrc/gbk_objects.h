@@ -56,6 +56,10 @@ struct gbk_matrix_s {
     gbk::SegCSR* seg[2] = {nullptr, nullptr};
     gbk::Lav* lav[2] = {nullptr, nullptr};
     gbk::Workspace ws;
+    // vertices with more in-edges than the BFS pull kernel gives one lane,
+    // listed on the first BFS (bfs_nheavy < 0: not yet)
+    int32_t* bfs_heavy = nullptr;
+    int64_t bfs_nheavy = -1;
 };
 
 struct gbk_vector_s {

@@ -32,6 +32,9 @@ static void free_matrix_storage(gbk_matrix A) {
     free_csr(A->csc);
     for (auto& b : A->ws.buf) { dev_free(b); b = nullptr; }
     A->ws.bytes = 0;
+    dev_free(A->bfs_heavy);
+    A->bfs_heavy = nullptr;
+    A->bfs_nheavy = -1;
     A->built = false;
 }
 
