@@ -240,29 +240,11 @@ __global__ void k_set_level(int32_t* level, int64_t s, int32_t* q, int32_t* coun
     *count = 1;
 }
 
-__global__ __launch_bounds__(256) void k_queue_edges(const int32_t* __restrict__ rp, const int32_t* __restrict__ q,
-                                                     int64_t nq, unsigned long long* __restrict__ sum) {
-    __shared__ unsigned long long s_d;
-    if (threadIdx.x == 0) s_d = 0;
-    __syncthreads();
-    unsigned long long d = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
-        const int u = q[i];
-        d += (unsigned long long)(rp[u + 1] - rp[u]);
-    }
-    // wave reduce, one shared atomic per wave, one global atomic per block
-    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-    if ((threadIdx.x & 63) == 0 && d) atomicAdd(&s_d, d);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_d) atomicAdd(sum, s_d);
-}
-
 struct BfsScratch {
     int32_t* q[2];
     int32_t* cnt;  // [2] queue counters, [2] heavy count
     int32_t* deg;
     int32_t* offs;
-    unsigned long long* esum;
     uint32_t* fb;     // frontier bitmap
     int32_t* heavy;   // vertices with more than BFS_HEAVY in-edges
 };
@@ -284,8 +266,7 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
     s.offs = s.deg + (n + 1);
     s.heavy = s.offs + (n + 1);
     s.cnt = s.heavy + (n + 1);
-    s.esum = (unsigned long long*)(((uintptr_t)(s.cnt + 4) + 15) & ~(uintptr_t)15);
-    s.fb = (uint32_t*)(((uintptr_t)(s.esum + 2) + 15) & ~(uintptr_t)15);
+    s.fb = (uint32_t*)(((uintptr_t)(s.cnt + 4) + 15) & ~(uintptr_t)15);
     GBK_TRY(fill_value(GBK_INT32, level, n, 0.0));
     hipLaunchKernelGGL(k_set_level, dim3(1), dim3(1), 0, st, level, source, s.q[0], s.cnt);
     GBK_LAUNCH_CHECK();
@@ -307,21 +288,21 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
     const int G = avg_in < 64 ? 1 : 4;
     const int heavy = BFS_HEAVY;
     while (nq > 0) {
-        // out-edges of the frontier
-        unsigned long long m_f = 0;
-        GBK_TRY(fill_value(GBK_INT64, s.esum, 1, 0.0));
-        hipLaunchKernelGGL(k_queue_edges, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1024)), dim3(256), 0, st,
-                           out.row_ptr, s.q[qi], nq, s.esum);
+        // out-degrees of the frontier and their prefix sums: the total is the
+        // frontier's out-edge count m_f (Beamer's switch), the offsets are
+        // the push step's edge-balanced expansion
+        hipLaunchKernelGGL(k_frontier_deg, grid_n(nq), dim3(256), 0, st, out.row_ptr, s.q[qi], nq, s.deg);
         GBK_LAUNCH_CHECK();
-        GBK_HIP(hipMemcpyAsync(&m_f, s.esum, sizeof(m_f), hipMemcpyDeviceToHost, st));
-        GBK_HIP(hipStreamSynchronize(st));
-        traversed += (int64_t)m_f;
+        int32_t ne = 0;
+        GBK_TRY(exclusive_scan_i32(s.deg, s.offs, nq, &ne));
+        const int64_t m_f = ne;
+        traversed += m_f;
         if (!pull && (double)m_f > (double)m_u / alpha) pull = true;
         else if (pull && (double)nq < (double)n / beta) pull = false;
-        m_u -= (int64_t)m_f;
+        m_u -= m_f;
         int32_t* nxt = s.q[qi ^ 1];
         int32_t* ncnt = s.cnt + (qi ^ 1);
-        GBK_TRY(fill_value(GBK_INT32, ncnt, 1, 0.0));
+        int64_t next_nq = 0;
         if (pull) {
             ++pulls;
             if (A->bfs_nheavy < 0) {
@@ -364,23 +345,21 @@ static int bfs_run(gbk_matrix A, int64_t source, int32_t* level, int64_t* stats)
             GBK_TRY(exclusive_scan_i32(s.deg, s.offs, nb, &total));
             hipLaunchKernelGGL(k_lvl_compact, dim3((unsigned)nb), dim3(256), 0, st, level, n, cur + 1, s.offs, nxt);
             GBK_LAUNCH_CHECK();
-            GBK_HIP(hipMemcpyAsync(ncnt, &total, sizeof(int32_t), hipMemcpyHostToDevice, st));
+            next_nq = total;
         } else {
             ++pushes;
-            hipLaunchKernelGGL(k_frontier_deg, grid_n(nq), dim3(256), 0, st, out.row_ptr, s.q[qi], nq, s.deg);
-            GBK_LAUNCH_CHECK();
-            int32_t ne = 0;
-            GBK_TRY(exclusive_scan_i32(s.deg, s.offs, nq, &ne));
+            GBK_TRY(fill_value(GBK_INT32, ncnt, 1, 0.0));
             if (ne > 0) {
                 hipLaunchKernelGGL(k_bfs_push, grid_n(ne), dim3(256), 0, st, out.row_ptr, out.col, s.q[qi], nq,
                                    s.offs, (int64_t)ne, level, cur + 1, nxt, ncnt);
                 GBK_LAUNCH_CHECK();
             }
+            int32_t c = 0;
+            GBK_HIP(hipMemcpyAsync(&c, ncnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            GBK_HIP(hipStreamSynchronize(st));
+            next_nq = c;
         }
-        int32_t c = 0;
-        GBK_HIP(hipMemcpyAsync(&c, ncnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        GBK_HIP(hipStreamSynchronize(st));
-        nq = c;
+        nq = next_nq;
         qi ^= 1;
         ++cur;
         ++levels;
