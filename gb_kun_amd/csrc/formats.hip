@@ -11,11 +11,12 @@
 // "problematic" chunks run on the row-split CSR kernel.
 //
 // Segmented CSR (spmv_seg.h:9-308) and LAV (spmv_lav.h:8-156): per-segment
-// sums over a virtual CSR of (segment, row) pairs on the merge-path kernel,
-// with the tiles ordered so that a segment's tiles are dispatched to one XCD
-// and its slice of x stays in that XCD's L2; then a per-row merge of the
-// pair sums in segment order (deterministic, no atomics).  LAV first gathers
-// x into the degree-sorted column order.
+// sums over a virtual CSR of (segment, row) pairs, each segment's work on one
+// XCD so its slice of x stays in that XCD's L2; then a per-row merge of the
+// pair sums in segment order (deterministic, no atomics).  The segmented
+// format runs k_seg_flag (pair ends flagged in the column words, the
+// segment's most referenced x values in LDS); LAV runs the merge-path kernel
+// over XCD-ordered tiles after gathering x into its degree-sorted order.
 #include <algorithm>
 #include <mutex>
 #include <vector>
