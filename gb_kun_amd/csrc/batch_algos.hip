@@ -508,7 +508,7 @@ class PprEngine : public Engine {
         set_thread_stream(prev);
         GBK_TRY(rc);
         const int per = 256 / kw;
-        const int64_t ub = std::max<int64_t>(std::min<int64_t>((n_ + per - 1) / per, (int64_t)device_cu_count() * 32), 1);
+        const int64_t ub = std::max<int64_t>(std::min<int64_t>((n_ + per - 1) / per, (int64_t)device_cu_count() * 4), 1);
         const int64_t sb = std::min<int64_t>(((int64_t)nsinks_ + per - 1) / per, (int64_t)device_cu_count() * 8);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink, 64);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.delta, 64);
