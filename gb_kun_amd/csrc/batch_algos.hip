@@ -418,6 +418,14 @@ __global__ void k_ppr_seed(float* __restrict__ R, int64_t n, int k, const int32_
     for (int i = 0; i < count; ++i) R[v * k + slot[i]] = (src[i] == v) ? 1.0f : 0.0f;
 }
 
+// A freshly seeded column is one-hot: its sink sum is 1 when its source is a
+// sink, else 0.
+__global__ void k_ppr_seed_sink(double* __restrict__ sink, const int32_t* __restrict__ sink_flag,
+                                const int32_t* __restrict__ src, const int32_t* __restrict__ slot, int count) {
+    const int i = threadIdx.x;
+    if (i < count) sink[slot[i]] = sink_flag[src[i]] ? 1.0 : 0.0;
+}
+
 template <typename T>
 __global__ void k_extract_col(const T* __restrict__ M, int64_t n, int k, int q, T* __restrict__ out) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -444,16 +452,17 @@ class PprEngine : public Engine {
         GBK_HIP(hipHostMalloc((void**)&hdelta_, sizeof(double) * 64, hipHostMallocDefault));
         char* base = (char*)(((uintptr_t)ws_ + 255) & ~(uintptr_t)255);
         L_ = ppr_layout(A_->csr, n_, k_, base);
+        // sink flags (the update pass forms the sink sums from them)
         hipLaunchKernelGGL(k_sink_flags, grid_b(n_), dim3(256), 0, st_, A_->csc.row_ptr, n_, L_.flag);
-        GBK_LAUNCH_CHECK();
-        GBK_TRY(exclusive_scan_i32_on(L_.flag, L_.offs, n_, &nsinks_, st_, L_.partial, L_.part_bytes));
-        hipLaunchKernelGGL(k_compact_flags, grid_b(n_), dim3(256), 0, st_, L_.flag, L_.offs, n_, L_.sinks);
         GBK_LAUNCH_CHECK();
         // every column starts frozen at zero
         std::vector<int32_t> ones(64, 1), zeros(64, 0);
         GBK_HIP(hipMemcpyAsync(L_.done, ones.data(), sizeof(int32_t) * 64, hipMemcpyHostToDevice, st_));
         GBK_HIP(hipMemcpyAsync(L_.src, zeros.data(), sizeof(int32_t) * 64, hipMemcpyHostToDevice, st_));
         hipLaunchKernelGGL(k_ppr_init, grid_b(n_ * k_), dim3(256), 0, st_, L_.R, n_, k_, L_.src);
+        GBK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink, 64);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink2, 64);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipStreamSynchronize(st_));
         for (int q = 0; q < 64; ++q) done_[q] = 1;
@@ -477,6 +486,8 @@ class PprEngine : public Engine {
             GBK_HIP(hipMemcpyAsync(seeds_, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice, st_));
             const int cnt = (int)pend_slot_.size();
             hipLaunchKernelGGL(k_ppr_seed, grid_b(n_), dim3(256), 0, st_, L_.R, n_, k_, seeds_, seeds_ + cnt, cnt);
+            GBK_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_ppr_seed_sink, dim3(1), dim3(64), 0, st_, L_.sink, L_.flag, seeds_, seeds_ + cnt, cnt);
             GBK_LAUNCH_CHECK();
             for (int i = 0; i < cnt; ++i) {
                 const int q = pend_slot_[i];
@@ -509,18 +520,17 @@ class PprEngine : public Engine {
         GBK_TRY(rc);
         const int per = 256 / kw;
         const int64_t ub = std::max<int64_t>(std::min<int64_t>((n_ + per - 1) / per, (int64_t)device_cu_count() * 4), 1);
-        const int64_t sb = std::min<int64_t>(((int64_t)nsinks_ + per - 1) / per, (int64_t)device_cu_count() * 8);
-        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink, 64);
+        // the sink sums of the current ranks are L_.sink (formed by the last
+        // update, or set when a column was seeded); the update forms the next
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.sink2, 64);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, L_.delta, 64);
-        if (nsinks_ > 0 && sb > 0)
-            hipLaunchKernelGGL(k_ppr_sink, dim3((unsigned)sb), dim3(256), 0, st_, L_.R, kw, k_, L_.sinks,
-                               (int64_t)nsinks_, L_.sink);
         hipLaunchKernelGGL(k_ppr_update, dim3((unsigned)ub), dim3(256), 0, st_, L_.R, L_.Y, L_.Y, n_, kw, k_, L_.src,
-                           L_.sink, L_.done, (float)damping_, L_.delta, (const int32_t*)nullptr, (double*)nullptr);
+                           L_.sink, L_.done, (float)damping_, L_.delta, L_.flag, L_.sink2);
         GBK_LAUNCH_CHECK();
         GBK_HIP(hipMemcpyAsync(hdelta_, L_.delta, sizeof(double) * kw, hipMemcpyDeviceToHost, st_));
         GBK_HIP(hipStreamSynchronize(st_));
         std::swap(L_.R, L_.Y);
+        std::swap(L_.sink, L_.sink2);
         bool changed = false;
         for (int q = 0; q < kw; ++q) {
             if (!((occupied_ >> q) & 1ull) || done_[q]) continue;
@@ -553,7 +563,6 @@ class PprEngine : public Engine {
     int32_t* seeds_ = nullptr;
     size_t ws_bytes_ = 0;
     PprLayout L_{};
-    int32_t nsinks_ = 0;
     double* hdelta_ = nullptr;
     uint64_t occupied_ = 0;
     int32_t done_[64];
