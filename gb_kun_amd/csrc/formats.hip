@@ -835,8 +835,7 @@ __global__ void k_flag_tiles(const int32_t* __restrict__ vrp, const int64_t* __r
 
 // The flagged layout from the hot encoding, the virtual row pointers and the
 // segments' pair / entry bounds.
-static int build_flagged(SegCSR& s, size_t vs, const std::vector<int32_t>& seg_pair,
-                         const std::vector<int32_t>& seg_k) {
+static int build_flagged(SegCSR& s, size_t vs, const std::vector<int32_t>& seg_k) {
     hipStream_t st = stream();
     SegHot& o = s.hot;
     const int64_t nnz = s.v.nnz, nseg = s.nseg;
@@ -860,7 +859,6 @@ static int build_flagged(SegCSR& s, size_t vs, const std::vector<int32_t>& seg_p
                        (int)nseg, o.nftiles, nnz, o.ftiles);
     GBK_LAUNCH_CHECK();
     GBK_TRY(dev_alloc(&o.fcarry, 2 * vs * (size_t)(o.nftiles > 0 ? o.nftiles : 1)));
-    (void)seg_pair;
     GBK_HIP(hipStreamSynchronize(st));
     return GBK_SUCCESS;
 }
@@ -880,7 +878,6 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
         }
         GBK_TRY(seg_pairs_device(m, type, nseg, key.as<uint32_t>(), nullptr, s, seg_pair));
     }
-    const int64_t npairs = s.npairs;
     // hot slots: columns by (segment, descending in-degree, id)
     SegHot& o = s.hot;
     const int H = hot_slots(vs);
@@ -940,7 +937,7 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
     GBK_TRY(upload(&o.seg_pair, seg_pair));
     if (n <= (int64_t)0x40000000) {
         // the flagged layout (columns below 2^30) replaces the column arrays
-        GBK_TRY(build_flagged(s, vs, seg_pair, seg_k));
+        GBK_TRY(build_flagged(s, vs, seg_k));
         GBK_TRY(build_block_pairs(s));
         GBK_HIP(hipStreamSynchronize(st));
         dev_free(o.hcol);
@@ -956,7 +953,6 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
         o.hcol = nullptr;
         GBK_TRY(seg_plain_tiles(s, type, seg_pair));
     }
-    (void)npairs;
     return GBK_SUCCESS;
 }
 
