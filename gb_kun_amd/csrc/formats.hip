@@ -73,8 +73,9 @@ struct SegCSR {
     int32_t* tile_map = nullptr;
     int32_t* rpp = nullptr;        // row -> its pairs (rpp, row_pairs), for the plain kernels
     int32_t* row_pairs = nullptr;
-    int32_t* pair_row = nullptr;   // pair -> row, and for every block of MERGE_ROWS rows and
-    int32_t* block_pair = nullptr; // segment its first pair (the per-row merge of CSR_SEG)
+    int32_t* pair_row = nullptr;   // pair -> row (while building)
+    uint16_t* pair_off = nullptr;  // pair -> row % MERGE_ROWS, and for every block of MERGE_ROWS
+    int32_t* block_pair = nullptr; // rows and segment its first pair (the per-row merge of CSR_SEG)
     void* partial = nullptr;
     int64_t n = 0;
     SegHot hot;  // built for CSR_SEG (hot.fcol == nullptr otherwise)
@@ -492,6 +493,7 @@ static void free_seg_fields(SegCSR& s) {
     dev_free(s.rpp);
     dev_free(s.row_pairs);
     dev_free(s.pair_row);
+    dev_free(s.pair_off);
     dev_free(s.block_pair);
     dev_free(s.partial);
     dev_free(s.hot.hcol);
@@ -773,6 +775,15 @@ __global__ void k_block_pairs(const int32_t* __restrict__ pair_row, const int32_
     block_pair[t] = (int32_t)lo;
 }
 
+static_assert((MERGE_ROWS & (MERGE_ROWS - 1)) == 0 && MERGE_ROWS <= 65536, "row offsets fit 16 bits");
+
+__global__ void k_pair_off(const int32_t* __restrict__ pair_row, int64_t npairs, uint16_t* __restrict__ pair_off) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < npairs) pair_off[p] = (uint16_t)(pair_row[p] & (MERGE_ROWS - 1));
+}
+
+// The per-row merge's tables: block_pair, and each pair's row as an offset
+// in its block of MERGE_ROWS rows (2 bytes a pair instead of 4).
 static int build_block_pairs(SegCSR& s) {
     const int64_t nblocks = (s.n + MERGE_ROWS - 1) / MERGE_ROWS;
     const int64_t m = (nblocks + 1) * s.nseg;
@@ -780,6 +791,11 @@ static int build_block_pairs(SegCSR& s) {
     if (m > 0) {
         hipLaunchKernelGGL(k_block_pairs, grid_e(m), dim3(256), 0, stream(), s.pair_row, s.hot.seg_pair, s.nseg,
                            nblocks, s.block_pair);
+        GBK_LAUNCH_CHECK();
+    }
+    GBK_TRY(dev_alloc((void**)&s.pair_off, 2 * (s.npairs > 0 ? s.npairs : 1)));
+    if (s.npairs > 0) {
+        hipLaunchKernelGGL(k_pair_off, grid_e(s.npairs), dim3(256), 0, stream(), s.pair_row, s.npairs, s.pair_off);
         GBK_LAUNCH_CHECK();
     }
     return GBK_SUCCESS;
@@ -948,6 +964,8 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
         s.rpp = nullptr;
         dev_free(s.row_pairs);
         s.row_pairs = nullptr;
+        dev_free(s.pair_row);
+        s.pair_row = nullptr;
     } else {
         dev_free(o.hcol);
         o.hcol = nullptr;
@@ -993,7 +1011,7 @@ __global__ void k_seg_merge(const int32_t* __restrict__ rpp, const int32_t* __re
 // order: the same fold as k_seg_merge, without the row -> pair indirection
 // and its dependent loads.
 template <typename T, int ADD>
-__global__ __launch_bounds__(MERGE_THREADS) void k_seg_merge_rows(const int32_t* __restrict__ pair_row,
+__global__ __launch_bounds__(MERGE_THREADS) void k_seg_merge_rows(const uint16_t* __restrict__ pair_off,
                                                                   const T* __restrict__ partial,
                                                                   const int32_t* __restrict__ block_pair, int nseg,
                                                                   int64_t n, T* __restrict__ y, int accum) {
@@ -1015,8 +1033,8 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_seg_merge_rows(const int32_t*
                 const int p = lo + (int)threadIdx.x + u * MERGE_THREADS;
                 const bool ok = p < hi;
                 const int pc = ok ? p : 0;  // an in-range index for the unused lanes
-                const int pr = pair_row[pc];
-                rr[q][u] = ok ? pr - (int)r0 : -1;
+                const int pr = pair_off[pc];
+                rr[q][u] = ok ? pr : -1;
                 pv[q][u] = partial[pc];
             }
         }
@@ -1315,7 +1333,7 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
     }
     if (s->n > 0) {
         hipLaunchKernelGGL((k_seg_merge_rows<T, ADD>), dim3((unsigned)((s->n + MERGE_ROWS - 1) / MERGE_ROWS)),
-                           dim3(MERGE_THREADS), 0, st, s->pair_row, (const T*)s->partial, s->block_pair, s->nseg, s->n,
+                           dim3(MERGE_THREADS), 0, st, s->pair_off, (const T*)s->partial, s->block_pair, s->nseg, s->n,
                            y, accum);
         GBK_LAUNCH_CHECK();
     }
@@ -1549,6 +1567,7 @@ int64_t format_bytes(gbk_matrix A) {
             if (sg->rpp) b += (sg->n + 1) * 4;
             if (sg->row_pairs) b += sg->npairs * 4;
             if (sg->pair_row) b += sg->npairs * 4;
+            if (sg->pair_off) b += sg->npairs * 2;
             if (sg->block_pair) b += ((sg->n + MERGE_ROWS - 1) / MERGE_ROWS + 1) * sg->nseg * 4;
         }
         if (sg && sg->hot.fcol)
