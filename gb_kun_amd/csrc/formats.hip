@@ -970,6 +970,9 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
         dev_free(o.hcol);
         o.hcol = nullptr;
         GBK_TRY(seg_plain_tiles(s, type, seg_pair));
+        GBK_HIP(hipStreamSynchronize(st));
+        dev_free(s.pair_row);
+        s.pair_row = nullptr;
     }
     return GBK_SUCCESS;
 }
