@@ -637,6 +637,23 @@ __global__ void k_hot_assign(const int32_t* __restrict__ order, const int32_t* _
     }
 }
 
+// The hot slots renumbered in column order inside each segment (the set of
+// hot columns is unchanged): pos is the exclusive prefix count of hot
+// columns, so the per-SpMV gather of their x values reads x ascending.
+__global__ void k_hot_flag(const int32_t* __restrict__ slot, int64_t n, int32_t* __restrict__ flag) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) flag[c] = slot[c] >= 0 ? 1 : 0;
+}
+__global__ void k_hot_reslot(const int32_t* __restrict__ pos, int64_t n, int64_t w, int H, int32_t* __restrict__ slot,
+                             int32_t* __restrict__ hot_ids) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || slot[c] < 0) return;
+    const int64_t sg = c / w;
+    const int32_t ns = pos[c] - pos[sg * w];
+    slot[c] = ns;
+    hot_ids[sg * H + ns] = (int32_t)c;
+}
+
 __global__ void k_hot_encode(const int32_t* __restrict__ vcol, const int32_t* __restrict__ slot, int64_t nnz,
                              int32_t* __restrict__ hcol) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -929,6 +946,15 @@ static int build_seg_device(const DevCSR& m, const int32_t* optr, int type, int6
             if (H > 0) {
                 hipLaunchKernelGGL(k_hot_assign, grid_e(n), dim3(256), 0, st, corder.as<int32_t>(),
                                    deg.as<int32_t>(), n, w, H, slot.as<int32_t>(), o.hot_ids);
+                GBK_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_hot_flag, grid_e(n), dim3(256), 0, st, slot.as<int32_t>(), n,
+                                   ckey_out.as<int32_t>());
+                GBK_LAUNCH_CHECK();
+                // the scan writes n + 1 entries (the total last): deg, no
+                // longer needed, has room for them
+                GBK_TRY(exclusive_scan_i32(ckey_out.as<int32_t>(), deg.as<int32_t>(), n, nullptr));
+                hipLaunchKernelGGL(k_hot_reslot, grid_e(n), dim3(256), 0, st, deg.as<int32_t>(), n, w, H,
+                                   slot.as<int32_t>(), o.hot_ids);
                 GBK_LAUNCH_CHECK();
             }
         }
