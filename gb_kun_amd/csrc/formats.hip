@@ -64,6 +64,8 @@ struct SegHot {
     int2* ftiles = nullptr;        // nftiles + 1 (the last = {nnz, 0})
     int64_t* fseg_tile = nullptr;  // nseg + 1
     void* fcarry = nullptr;        // 2 * nftiles values
+    int32_t* fback = nullptr;      // nftiles: tiles back to the start of the pair that began
+                                   // earlier and ends in tile t (-1: none)
 };
 
 struct SegCSR {
@@ -504,6 +506,7 @@ static void free_seg_fields(SegCSR& s) {
     dev_free(s.hot.ftiles);
     dev_free(s.hot.fseg_tile);
     dev_free(s.hot.fcarry);
+    dev_free(s.hot.fback);
     s = SegCSR();
 }
 
@@ -868,6 +871,32 @@ __global__ void k_flag_tiles(const int32_t* __restrict__ vrp, const int64_t* __r
 
 // The flagged layout from the hot encoding, the virtual row pointers and the
 // segments' pair / entry bounds.
+// fback[t]: for the pair that began in an earlier tile and ends in tile t,
+// the number of tiles t - 1, t - 2, ... down to the one holding its first
+// entry (the tiles whose trailing sums k_flag_fix adds); -1 when no such
+// pair ends in t.
+__global__ void k_flag_back(const int32_t* __restrict__ vrp, const int2* __restrict__ ftiles, int64_t ntiles,
+                            int32_t* __restrict__ fback) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int32_t b = -1;
+    const int2 ti = ftiles[t];
+    if (ti.y < 0) {
+        const int p = ti.y & 0x7fffffff;
+        if (vrp[p + 1] <= ftiles[t + 1].x) {
+            const int start = vrp[p];
+            int64_t u = t - 1;
+            if (u >= 0) {
+                while (u > 0 && ftiles[u].x > start) --u;
+                b = (int32_t)(t - u);
+            } else {
+                b = 0;
+            }
+        }
+    }
+    fback[t] = b;
+}
+
 static int build_flagged(SegCSR& s, size_t vs, const std::vector<int32_t>& seg_k) {
     hipStream_t st = stream();
     SegHot& o = s.hot;
@@ -892,6 +921,12 @@ static int build_flagged(SegCSR& s, size_t vs, const std::vector<int32_t>& seg_k
                        (int)nseg, o.nftiles, nnz, o.ftiles);
     GBK_LAUNCH_CHECK();
     GBK_TRY(dev_alloc(&o.fcarry, 2 * vs * (size_t)(o.nftiles > 0 ? o.nftiles : 1)));
+    GBK_TRY(dev_alloc((void**)&o.fback, 4 * (size_t)(o.nftiles > 0 ? o.nftiles : 1)));
+    if (o.nftiles > 0) {
+        hipLaunchKernelGGL(k_flag_back, grid_e(o.nftiles), dim3(256), 0, st, s.v.row_ptr, o.ftiles, o.nftiles,
+                           o.fback);
+        GBK_LAUNCH_CHECK();
+    }
     GBK_HIP(hipStreamSynchronize(st));
     return GBK_SUCCESS;
 }
@@ -1269,37 +1304,29 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
 constexpr int FIX_STEPS = 8;
 
 template <typename T, int ADD>
-__global__ __launch_bounds__(256) void k_flag_fix(const int32_t* __restrict__ vrp, const int2* __restrict__ ftiles,
+__global__ __launch_bounds__(256) void k_flag_fix(const int32_t* __restrict__ fback, const int2* __restrict__ ftiles,
                                                   int64_t ntiles, const T* __restrict__ carry,
                                                   T* __restrict__ partial) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const T ident = add_identity<ADD, T>();
-    bool open = false;  // the walk continues below tile `next`
-    int p = 0, start = 0;
-    int64_t next = 0;
+    bool open = false;  // the walk continues from tile `next` down to tile `lo`
+    int p = 0;
+    int64_t next = 0, lo = 0;
     T total = ident;
     if (t < ntiles) {
-        const int2 ti = ftiles[t];
-        if (ti.y < 0) {
-            p = ti.y & 0x7fffffff;
-            if (vrp[p + 1] <= ftiles[t + 1].x) {  // the pair ends in this tile
-                start = vrp[p];
-                total = carry[2 * t + 1];
-                int64_t u = t - 1;
-                bool done = false;
-                for (int k = 0; k < FIX_STEPS && u >= 0; ++k, --u) {
-                    total = add_op<ADD, T>(carry[2 * u], total);
-                    if (ftiles[u].x <= start) {
-                        done = true;
-                        break;
-                    }
-                }
-                if (done || u < 0) partial[p] = total;
-                else {
-                    open = true;
-                    next = u;
-                }
+        const int b = fback[t];
+        if (b >= 0) {  // a pair that began earlier ends in this tile
+            p = ftiles[t].y & 0x7fffffff;
+            total = carry[2 * t + 1];
+#pragma unroll
+            for (int k = 0; k < FIX_STEPS; ++k)
+                if (k < b) total = add_op<ADD, T>(carry[2 * (t - 1 - k)], total);
+            if (b <= FIX_STEPS) partial[p] = total;
+            else {
+                open = true;
+                next = t - 1 - FIX_STEPS;
+                lo = t - b;
             }
         }
     }
@@ -1307,20 +1334,15 @@ __global__ __launch_bounds__(256) void k_flag_fix(const int32_t* __restrict__ vr
     while (todo) {
         const int src = __ffsll((long long)todo) - 1;
         todo &= todo - 1;
-        const int st = __shfl(start, src, 64);
         const int64_t hi0 = (int64_t)__shfl((int)next, src, 64);
+        const int64_t lo0 = (int64_t)__shfl((int)lo, src, 64);
         T acc = hot_shfl(total, src);
-        for (int64_t hi = hi0; hi >= 0; hi -= 64) {
+        for (int64_t hi = hi0; hi >= lo0; hi -= 64) {
             const int64_t u = hi - lane;
-            const bool in = u >= 0;
-            const int k0 = in ? ftiles[u].x : 0;
-            const uint64_t stop = __ballot(in && k0 <= st);  // lanes at or past the pair start
-            const int last = stop ? __ffsll((long long)stop) - 1 : 63;
-            T v = (in && lane <= last) ? carry[2 * u] : ident;
+            T v = u >= lo0 ? carry[2 * u] : ident;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v = add_op<ADD, T>(v, hot_shfl(v, lane ^ off));
             acc = add_op<ADD, T>(v, acc);
-            if (stop) break;
         }
         if (lane == src) partial[p] = acc;
     }
@@ -1357,7 +1379,7 @@ static int seg_hot_spmv(const SegCSR* s, const T* x, T* y, int accum) {
                            h.fseg_tile, (T*)h.fcarry);
         GBK_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_flag_fix<T, ADD>), dim3((unsigned)((h.nftiles + 255) / 256)), dim3(256), 0, st,
-                           s->v.row_ptr, h.ftiles, h.nftiles, (const T*)h.fcarry, (T*)s->partial);
+                           h.fback, h.ftiles, h.nftiles, (const T*)h.fcarry, (T*)s->partial);
         GBK_LAUNCH_CHECK();
     }
     if (s->n > 0) {
@@ -1600,7 +1622,7 @@ int64_t format_bytes(gbk_matrix A) {
             if (sg->block_pair) b += ((sg->n + MERGE_ROWS - 1) / MERGE_ROWS + 1) * sg->nseg * 4;
         }
         if (sg && sg->hot.fcol)
-            b += (int64_t)(sg->v.nnz * 4 + (int64_t)sg->nseg * sg->hot.H * (4 + vs) + sg->hot.nftiles * (8 + 2 * vs) +
+            b += (int64_t)(sg->v.nnz * 4 + (int64_t)sg->nseg * sg->hot.H * (4 + vs) + sg->hot.nftiles * (12 + 2 * vs) +
                            (sg->nseg + 1) * 16);
         if (A->lav[t]) b += A->lav[t]->seg.n * (4 + vs);
     }
