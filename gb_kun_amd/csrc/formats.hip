@@ -551,6 +551,9 @@ static int64_t seg_cols_default(int64_t n) {
 // Persistent hot-column kernel shape: 16 waves per block, the block's LDS
 // holding the hot x values of its segment.
 constexpr int HOT_WAVES = 16;
+// 16-byte pieces of hot x values each thread loads at once while its block
+// fills the LDS (160 KiB = 10 per thread of a 1024-thread block).
+constexpr int HOT_FILL = 10;
 // Flagged tiles: FLAG_EPL consecutive entries per lane of a wave.
 constexpr int FLAG_EPL = 4;
 constexpr int FLAG_CHUNK = 64 * FLAG_EPL;
@@ -1190,8 +1193,24 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
     const T ident = add_identity<ADD, T>();
     const uint64_t below = (1ull << lane) - 1ull;
     for (int seg = blockIdx.x & 7; seg < nseg; seg += 8) {
-        const T* src = xh + (int64_t)seg * H;
-        for (int i = threadIdx.x; i < H; i += HOT_WAVES * 64) s_x[i] = src[i];
+        {
+            // the segment's hot x values into LDS in 16-byte pieces (H is a
+            // multiple of 64 slots), all of a thread's loads issued before
+            // its first LDS write
+            const gbk_u32x4* src = (const gbk_u32x4*)(xh + (int64_t)seg * H);
+            gbk_u32x4* dst = (gbk_u32x4*)s_x;
+            constexpr int ROUND = HOT_FILL * HOT_WAVES * 64;
+            const int nq = (int)((int64_t)H * (int64_t)sizeof(T) / 16);
+            const int full = nq / ROUND * ROUND;
+            for (int q0 = threadIdx.x; q0 < full; q0 += ROUND) {
+                gbk_u32x4 v[HOT_FILL];
+#pragma unroll
+                for (int u = 0; u < HOT_FILL; ++u) v[u] = src[q0 + u * HOT_WAVES * 64];
+#pragma unroll
+                for (int u = 0; u < HOT_FILL; ++u) dst[q0 + u * HOT_WAVES * 64] = v[u];
+            }
+            for (int q = full + (int)threadIdx.x; q < nq; q += HOT_WAVES * 64) dst[q] = src[q];
+        }
         __syncthreads();
         const int64_t t_lo = fseg_tile[seg], t_hi = fseg_tile[seg + 1];
         const int64_t step = (int64_t)nbx * HOT_WAVES;
