@@ -1315,7 +1315,8 @@ __global__ __launch_bounds__(HOT_WAVES * 64) void k_seg_flag(
 
 // Pairs that began in an earlier tile and end in tile t: the deferred part
 // plus the trailing sums of the tiles back to the one holding the pair start,
-// added from the nearest tile back.  A thread walks up to FIX_STEPS tiles
+// added from the nearest tile back; how many tiles that is was stored per
+// tile at build time (fback, k_flag_back).  A thread walks up to FIX_STEPS tiles
 // back itself (most pairs span one or two tiles); the walks that go further
 // (pairs of hub rows) are finished by the whole wave, one at a time, 64
 // earlier tiles per round (lane l: tile u - l), each round's sums combined by
